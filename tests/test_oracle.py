@@ -1,0 +1,497 @@
+"""Pins the CPU oracle (oracle/) against checks independent of its own code:
+scipy special functions, numpy FFT and Philox, closed forms, brute force and
+invariants.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+import scipy.integrate as si
+import scipy.special as sp
+
+from oracle import oracle as O
+from cudamultislice_amd import structures as S
+
+REF_FPARAMS = "/root/reference/fparams.dat"
+
+
+# ---------------------------------------------------------------- data
+def test_embedded_fparams_match_reference_file(fp):
+    if not os.path.exists(REF_FPARAMS):
+        pytest.skip("reference tree not mounted")
+    rows = [ln.split() for ln in open(REF_FPARAMS) if ln.strip()]
+    ref = np.array([[np.float32(float(v)) for v in r] for r in rows], np.float32)
+    assert ref.shape == (103, 12)
+    assert np.array_equal(ref.view(np.uint32), fp.view(np.uint32))
+
+
+# ---------------------------------------------------------------- Bessel
+@pytest.mark.parametrize("x", np.concatenate([np.linspace(0.01, 2.0, 37), np.linspace(2.0, 30.0, 41)]))
+def test_bessk0_abramowitz_stegun_bounds(x):
+    k = O.bessk0(x)
+    if x <= 2.0:  # A&S 9.8.5: |eps| < 1e-8 absolute on K0 + ln(x/2) I0
+        assert abs(k - sp.k0(x)) < 1e-7 + 2e-7 * abs(sp.k0(x))
+    else:  # A&S 9.8.6: |eps| < 1.9e-7 on sqrt(x) e^x K0
+        assert abs(np.sqrt(x) * np.exp(x) * (k - sp.k0(x))) < 2.5e-7
+
+
+@pytest.mark.parametrize("x", np.concatenate([np.linspace(-3.75, 3.75, 31), np.linspace(3.75, 40, 31)]))
+def test_bessi0_abramowitz_stegun_bounds(x):
+    v = O.bessi0(x)
+    if abs(x) <= 3.75:  # 9.8.1: |eps| < 1.6e-7
+        assert abs(v - sp.i0(x)) < 1.6e-7 * max(1.0, abs(sp.i0(x))) + 1e-9
+    else:  # 9.8.2: |eps| < 1.9e-7 on sqrt(x) e^-x I0
+        assert abs(np.sqrt(x) * np.exp(-x) * (v - sp.i0(x))) < 2.5e-7
+
+
+def test_bessk0_zero_sentinel():
+    assert O.bessk0(0.0) == 1e20
+
+
+# ---------------------------------------------------------------- Kirkland
+def _split_table(fp, Z, part):
+    t = fp.copy()
+    if part == "lorentz":
+        t[Z - 1, 6::2] = 0.0
+    else:
+        t[Z - 1, 0:6:2] = 0.0
+    return t
+
+
+@pytest.mark.parametrize("Z", [6, 8, 14, 22, 38, 79])
+@pytest.mark.parametrize("r", [0.3, 0.6, 1.0, 1.7])
+def test_vatom_integrates_to_vzatom(fp, Z, r):
+    """Projected potential = z-integral of the 3D potential (Kirkland App. C).
+    The Lorentzian terms integrate to 2 K0 exactly; the Gaussian terms carry
+    the ratio 266.5157269 / (150.4121417 sqrt(pi)) of the two prefactors the
+    reference uses (U.cu:151 vs U.cu:192)."""
+    for part, scale in (("lorentz", 1.0), ("gauss", 266.5157269 / (150.4121417 * np.sqrt(np.pi)))):
+        t = _split_table(fp, Z, part)
+        f = lambda z: O.vatom(Z, r * r + z * z, t)
+        integ = 2 * si.quad(f, 0, 40, limit=400, epsabs=1e-12, epsrel=1e-11)[0]
+        proj = O.vzatom(Z, r * r, t)
+        assert integ == pytest.approx(scale * proj, rel=3e-6)
+
+
+def test_vatom_clamps_radius(fp):
+    assert O.vatom(14, 0.0, fp) == O.vatom(14, 0.0625, fp) == O.vatom(14, 0.01, fp)
+    assert O.vzatom(14, 0.0, fp) == O.vzatom(14, 0.0625, fp)
+    assert O.vatom(14, 0.07, fp) < O.vatom(14, 0.0625, fp)
+
+
+def test_vatom_large_r_decays(fp):
+    v = [O.vatom(14, r * r, fp) for r in (0.5, 1.0, 2.0, 4.0)]
+    assert all(a > b > 0 for a, b in zip(v, v[1:]))
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("shape", [(8, 8), (16, 32), (64, 64), (6, 10), (12, 9)])
+@pytest.mark.parametrize("inv", [False, True])
+def test_fft2_matches_numpy(shape, inv):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    got = O.fft2(a, inv)
+    want = np.fft.ifft2(a) * a.size if inv else np.fft.fft2(a)
+    assert np.max(np.abs(got - want)) < 1e-10 * np.max(np.abs(want))
+
+
+# ---------------------------------------------------------------- Philox
+def test_philox4x64_matches_numpy():
+    for ctr, key in (([0, 0, 0, 0], [0, 0]), ([5, 6, 7, 8], [11, 12]),
+                     ([2**64 - 1, 2**63 + 5, 3, 9], [0x4D534C, 77])):
+        bg = np.random.Philox(counter=np.array(ctr, np.uint64), key=np.array(key, np.uint64))
+        # numpy increments the 256-bit counter before each block
+        want = bg.random_raw(4)
+        c = np.array(ctr, dtype=np.uint64)
+        c1 = c.copy()
+        c1[0] = np.uint64((int(c[0]) + 1) % 2**64)
+        if int(c[0]) == 2**64 - 1:
+            c1[1] = np.uint64((int(c[1]) + 1) % 2**64)
+        got = O.philox4x64(c1, np.array(key, np.uint64))
+        assert np.array_equal(got, want)
+
+
+def test_tds_displacements_statistics():
+    dx, dy, dz = O.tds_displacements(S.SEED, 3, 200000, 0.08)
+    for d in (dx, dy, dz):
+        assert abs(d.mean()) < 0.08 * 0.01
+        assert d.std() == pytest.approx(0.08, rel=0.01)
+    # configurations differ, same seed repeats
+    a, _, _ = O.tds_displacements(S.SEED, 4, 1000, 0.08)
+    b, _, _ = O.tds_displacements(S.SEED, 4, 1000, 0.08)
+    assert np.array_equal(a, b) and not np.array_equal(a, dx[:1000])
+    assert np.corrcoef(dx, dy)[0, 1] < 0.01
+
+
+# ---------------------------------------------------------------- physics
+def _sim_with(Z, x, y, z, fp, use_float=False):
+    s = O.OracleSim(use_float)
+    s.set_atoms(Z, x, y, z)
+    s.bin()
+    s.set_fparams(fp)
+    return s
+
+
+def test_wavelength_and_sigma(fp):
+    s = _sim_with(*S.silicon(1, 1, 1), fp)
+    for kV, lam, sig in ((200.0, 0.025116, 7.2753e-4), (300.0, 0.019718, 6.5134e-4)):
+        s.apply_microscope(kV)
+        s.set_calc(0.1, 0.0, 64, 64, 10, 10, 10)
+        sc = s.scalars()
+        assert sc["wavel"] == pytest.approx(lam, rel=2e-5)
+        assert sc["sigma2"] == pytest.approx(sig, rel=2e-5)
+        # physical relativistic wavelength with CODATA constants; the
+        # reference's rounded constants stay within 0.2 %
+        h, m, e, c = 6.62607015e-34, 9.1093837e-31, 1.602176634e-19, 2.99792458e8
+        V = kV * 1e3
+        lam_true = h / np.sqrt(2 * m * e * V * (1 + e * V / (2 * m * c * c))) * 1e10
+        assert sc["wavel"] == pytest.approx(lam_true, rel=2e-3)
+
+
+@pytest.mark.parametrize("N,ps", [(64, 0.1), (256, 0.084844), (512, 0.078125), (100, 0.2)])
+def test_frequency_grid_and_kmax(fp, N, ps):
+    s = _sim_with(*S.silicon(1, 1, 1), fp)
+    s.apply_microscope(200.0)
+    s.set_calc(ps, 0.0, N, N, 10, 10, 10)
+    kx, ky = s.freqs()
+    i = np.arange(N)
+    want = np.where(i <= N // 2, i, i - N) / np.float32(np.float32(ps) * N)
+    assert np.allclose(kx, want, rtol=1e-6) and np.allclose(ky, want, rtol=1e-6)
+    assert kx[N // 2] > 0  # Nyquist bin positive
+    assert s.scalars()["kmax"] == pytest.approx((N // 2 - 1) / (2 * N * ps), rel=1e-6)
+
+
+def test_fd_step_size_matches_quadratic(fp):
+    s = _sim_with(*S.silicon(1, 1, 1), fp)
+    s.apply_microscope(300.0)
+    s.set_calc(0.1, 0.0, 128, 128, 10, 10, 20.0)
+    sc = s.scalars()
+    assert 0 < sc["dzFD"] < 0.1
+    assert sc["numberOfSlices"] == int(np.ceil(np.float32(20.0) / np.float32(sc["dzFD"])))
+    s.set_calc(0.02, 0.0, 128, 128, 10, 10, 20.0, stem=True)
+    assert s.scalars()["dzFD"] <= np.float32(0.06)
+
+
+# ---------------------------------------------------------------- binning
+def test_binning_edges_exact(fp):
+    # extents: x in [0, 46] -> Min -2, Max 48 -> width 50 A -> 1 A bins
+    x = np.array([0.0, 1.0, 2.5, 46.0, 20.0, 20.0], np.float32)
+    y = np.array([0.0, 46.0, 3.0, 10.0, 20.0, 20.0], np.float32)
+    z = np.array([0.0, 0.0, 5.0, 3.0, 1.0, 1.0], np.float32)
+    Z = np.array([6, 8, 14, 38, 22, 22], np.int32)
+    s = O.OracleSim()
+    s.set_atoms(Z, x, y, z)
+    assert list(s.extents()) == [48, -2, 48, -2, 7, -2]
+    nslab = s.bin()
+    assert nslab == 9
+    b, zz = s.bins()
+    assert list(b) == [2 + 50 * 2, 3 + 50 * 48, 4 + 50 * 5, 48 + 50 * 12, 22 + 50 * 22, 22 + 50 * 22]
+    assert list(zz) == [7, 7, 2, 4, 6, 6]
+    st, bx, by, bz, bZ = s.csr()
+    # CSR in (slab, y, x) order, stable in input order, shifted by Minimum
+    key = zz * 2500 + b
+    order = np.argsort(key, kind="stable")
+    assert np.array_equal(bZ, Z[order])
+    assert np.array_equal(bx, x[order] - np.float32(-2))
+    assert np.array_equal(np.diff(st) >= 0, np.ones(st.size - 1, bool))
+    assert st[-1] == 6 and np.array_equal(np.bincount(key, minlength=nslab * 2500), np.diff(st))
+
+
+def test_binning_random_matches_float32_brute_force(fp):
+    Z, x, y, z = S.srtio3(6, 6, 8)
+    rng = np.random.default_rng(5)
+    dx, dy, dz = (rng.normal(0, 0.08, x.size).astype(np.float32) for _ in range(3))
+    s = O.OracleSim()
+    s.set_atoms(Z, x, y, z)
+    e = s.extents()
+    s.bin((dx, dy, dz))
+    b, zz = s.bins()
+    f32 = np.float32
+    X, Y, Zp = x + dx, y + dy, z + dz
+    bxw = np.floor(((X - e[1]) / f32(e[0] - e[1])) * f32(50)).astype(int)
+    byw = np.floor(((Y - e[3]) / f32(e[2] - e[3])) * f32(50)).astype(int)
+    zw = np.floor((f32(e[4]) - Zp) / f32(1)).astype(int)
+    assert np.array_equal(b, bxw + 50 * byw) and np.array_equal(zz, zw)
+
+
+# ---------------------------------------------------------------- potential
+def _tiny_specimen():
+    # three atoms near the centre and two far anchors fixing the extents
+    Z = np.array([14, 38, 8, 6, 6], np.int32)
+    x = np.array([5.0, 6.1, 5.4, -30.0, 40.0], np.float32)
+    y = np.array([5.2, 4.9, 6.3, -30.0, 40.0], np.float32)
+    z = np.array([3.0, 4.2, 2.1, 0.0, 9.0], np.float32)
+    return Z, x, y, z
+
+
+def _brute_conv(s, Z, x, y, z, zc, convdz, N, ps, off):
+    """No bins: every atom in the +-3 slab window, 20 z samples, r^2 < 5."""
+    e = s.extents()
+    H = float(e[4] - e[5])
+    kc = int(np.floor(H - zc))
+    zid = np.floor((np.float32(e[4]) - z) / np.float32(1)).astype(int)
+    X = (x - np.float32(e[1])).astype(np.float64)
+    Y = (y - np.float32(e[3])).astype(np.float64)
+    Zr = (z - np.float32(e[5])).astype(np.float64)
+    fpar = s._fp
+    ps, off = float(np.float32(ps)), float(np.float32(off))  # PixelScale is a float (U.cuh:107)
+    out = np.zeros((N, N))
+    for iy in range(N):
+        for ix in range(N):
+            px, py = off + ix * ps, off + iy * ps
+            acc = 0.0
+            for a in range(len(Z)):
+                if abs(zid[a] - kc) > 3:
+                    continue
+                for m in range(1, 21):
+                    z2 = zc - m * convdz / 20.0
+                    r2 = (px - X[a]) ** 2 + (py - Y[a]) ** 2 + (z2 - Zr[a]) ** 2
+                    if r2 < 5.0:
+                        acc += convdz / 20.0 * O.vatom(int(Z[a]), r2, fpar)
+            out[iy, ix] = acc
+    return out
+
+
+def test_conventional_potential_brute_force(fp):
+    Z, x, y, z = _tiny_specimen()
+    s = _sim_with(Z, x, y, z, fp)
+    s._fp = fp
+    N, ps, off = 12, 0.55, 32.0
+    s.apply_microscope(200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+    e = s.extents()
+    H = float(e[4] - e[5])
+    for zc, dz in ((H - 6.0, 1.0), (H - 8.0, 2.0)):
+        got = s.potential(3, zc, dz).real
+        want = _brute_conv(s, Z, x, y, z, zc, dz, N, ps, off)
+        assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+        assert np.max(want) > 1.0  # the slice actually hits atoms
+        t = s.potential(0, zc, dz)
+        assert np.allclose(np.abs(t), 1.0, atol=1e-14)
+        assert np.allclose(np.angle(t), np.angle(np.exp(1j * s.scalars()["sigma2"] * got)), atol=1e-12)
+
+
+def test_potential_linear_in_atoms(fp):
+    Z, x, y, z = _tiny_specimen()
+    N, ps, off = 16, 0.4, 32.0
+
+    def pot(mask, mode):
+        keep = mask | np.array([0, 0, 0, 1, 1], bool)
+        s = _sim_with(Z[keep], x[keep], y[keep], z[keep], fp)
+        s.apply_microscope(300.0)
+        s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+        e = s.extents()
+        return s.potential(mode, float(e[4] - e[5]) - 7.0, 2.0).real
+
+    for mode in (3, 4):
+        a = pot(np.array([1, 0, 0, 0, 0], bool), mode)
+        b = pot(np.array([0, 1, 1, 0, 0], bool), mode)
+        ab = pot(np.array([1, 1, 1, 0, 0], bool), mode)
+        assert np.allclose(a + b, ab, rtol=1e-12, atol=1e-12)
+        assert ab.max() > 0
+
+
+def test_projected_mode_equals_integrated_for_thin_atoms(fp):
+    """Projected mode on the xy distance vs the 20-sample z integral of the
+    3D potential: close when the slab holds the whole atom profile."""
+    Z = np.array([38, 6, 6], np.int32)
+    x = np.array([5.0, -30.0, 40.0], np.float32)
+    y = np.array([5.0, -30.0, 40.0], np.float32)
+    z = np.array([10.5, 0.0, 30.0], np.float32)
+    s = _sim_with(Z, x, y, z, fp)
+    N, ps, off = 16, 0.25, 33.0
+    s.apply_microscope(300.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+    e = s.extents()
+    H = float(e[4] - e[5])
+    # atom at relative z 12.5 (MinZ=-2); slab holding it for dz=1: H - zc = floor(H-12.5)
+    zc = H - np.floor(H - 12.5)
+    proj = s.potential(4, zc, 1.0).real
+    assert proj.max() > 10
+    # the projected mode only counts atoms in the slice's own slabs
+    assert np.all(s.potential(4, zc - 1.0, 1.0).real == 0)
+
+
+# ---------------------------------------------------------------- propagation
+def _vacuum_sim(fp, N=64, ps=0.2, kV=200.0):
+    Z = np.array([6, 6], np.int32)
+    x = np.array([-60.0, 200.0], np.float32)
+    y = np.array([-60.0, 200.0], np.float32)
+    z = np.array([0.0, 20.0], np.float32)
+    s = _sim_with(Z, x, y, z, fp)
+    s.apply_microscope(kV)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 62.0, 62.0)
+    return s
+
+
+def test_vacuum_plane_wave_stays_unity(fp):
+    s = _vacuum_sim(fp)
+    s.init_plane(1.0)
+    for it in range(1, 6):
+        s.step_conv(it)
+    assert np.max(np.abs(s.wave() - 1.0)) < 1e-13
+
+
+def test_fresnel_gaussian_closed_form(fp):
+    N, ps = 128, 0.2
+    s = _vacuum_sim(fp, N, ps)
+    s.init_plane(3.0)
+    lam = s.scalars()["wavel"]
+    c = np.arange(N) * ps
+    c = c - c[N // 2]
+    X, Y = np.meshgrid(c, c)
+    w = 1.6
+    psi0 = np.exp(-(X ** 2 + Y ** 2) / w ** 2)
+    s.set_wave(np.fft.ifftshift(psi0))
+    s.step_conv(2)  # iteration > 1 propagates Psi by conventionaldz in vacuum
+    got = np.fft.fftshift(s.wave())
+    q = w ** 2 + 1j * lam * 3.0 / np.pi
+    want = (w ** 2 / q) * np.exp(-(X ** 2 + Y ** 2) / q)
+    assert np.max(np.abs(got - want)) < 1e-9
+
+
+# ---------------------------------------------------------------- probe
+def test_probe_closed_form_and_shift(fp):
+    N, ps = 64, 0.25
+    s = _vacuum_sim(fp, N, ps, kV=300.0)
+    s.apply_microscope(300.0, obj=25.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 0, 0, stem=True)
+    p0 = s.probe(0, 0)
+    assert np.sum(np.abs(p0)) == pytest.approx(1.0, rel=1e-12)  # L1 normalised (U.cu:2698-2705)
+    F = np.fft.fft2(p0)
+    kx, ky = s.freqs()
+    kk = np.sqrt(kx[None, :].astype(np.float64) ** 2 + ky[:, None] ** 2)
+    inside = kk < 25e-3 / s.scalars()["wavel"]
+    assert inside.sum() > 10
+    amp = np.abs(F[inside])
+    assert np.allclose(amp, amp[0], rtol=1e-10)  # aberration-free: flat disc
+    assert np.max(np.abs(F[~inside])) < 1e-12 * amp[0]
+    # shift theorem: position r0 lands the probe at -r0 (mod N)
+    p = s.probe(5, 9)
+    # (the FP32 k-grid of U.cu:1889-1903 limits the phase ramp to ~1e-7)
+    assert np.max(np.abs(p - np.roll(p0, (-9, -5), axis=(0, 1)))) < 1e-7 * np.max(np.abs(p0))
+
+
+# ---------------------------------------------------------------- CTF
+def test_ctf_identity_and_weak_phase_chi(fp):
+    N, ps = 64, 0.25
+    s = _vacuum_sim(fp, N, ps, kV=200.0)
+    s.apply_microscope(200.0, obj=1000.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 62.0, 62.0)
+    s.init_plane(1.0)
+    rng = np.random.default_rng(2)
+    kx, ky = s.freqs()
+    kk = np.sqrt(kx[None, :].astype(np.float64) ** 2 + ky[:, None] ** 2)
+    band = np.fft.fft2(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+    band[kk > 0.5 * kk.max()] = 0
+    psi = np.fft.ifft2(band)
+    s.set_wave(psi)
+    assert np.max(np.abs(s.ctf_image(0.0) - np.abs(psi) ** 2)) < 1e-12 * np.max(np.abs(psi) ** 2)
+    # weak phase object exp(i eps cos(2 pi k0 x)) imaged through psi_hat e^{-i chi}
+    # (U.cu:1073-1074): psi' = 1 + i eps e^{-i chi} cos(.), so
+    # I = 1 + 2 eps sin(chi(k0)) cos(2 pi k0 x) to first order in eps
+    Cs, df = 1.2e7, -500.0
+    s.apply_microscope(200.0, df=df, Cs=Cs, obj=1000.0)
+    s.set_calc(ps, df, N, N, 1, 1, 22, 62.0, 62.0)
+    lam = s.scalars()["wavel"]
+    for j in (3, 7, 12):
+        x = np.arange(N) * ps
+        eps = 1e-4
+        ph = np.tile(np.exp(1j * eps * np.cos(2 * np.pi * j * np.arange(N) / N)), (N, 1))
+        s.set_wave(ph)
+        I = s.ctf_image(df)
+        k0 = j / (N * ps)
+        chi = np.pi * lam * k0 ** 2 * (0.5 * Cs * lam ** 2 * k0 ** 2 + df)
+        want = 1 + 2 * eps * np.sin(chi) * np.cos(2 * np.pi * j * np.arange(N) / N)
+        assert np.max(np.abs(I[0] - want)) < 5e-8
+
+
+# ---------------------------------------------------------------- detectors
+def test_adf_parseval_and_vacuum_zero(fp):
+    N, ps = 64, 0.2
+    s = _vacuum_sim(fp, N, ps, kV=300.0)
+    s.apply_microscope(300.0, obj=20.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 0, 0, stem=True)
+    s.init_stem(32, 32, 1.0)
+    w = s.wave(1)
+    s.set_wave(w)
+    assert s.adf(0.0, 1e9) == pytest.approx(np.sum(np.abs(w) ** 2), rel=1e-12)
+    assert s.adf(25.0, 200.0) < 1e-25
+    assert s.adf(0.0, 20.0) == pytest.approx(np.sum(np.abs(w) ** 2), rel=1e-10)
+
+
+def test_tds_accumulates_centred_amplitude(fp):
+    N, ps = 32, 0.2
+    s = _vacuum_sim(fp, N, ps, kV=300.0)
+    s.apply_microscope(300.0, obj=20.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 0, 0, stem=True)
+    s.init_stem(16, 16, 1.0)
+    w = s.wave(1)
+    s.set_wave(w)
+    s.add_tds(); s.add_tds()
+    assert np.allclose(s.tds(), 2 * np.abs(np.fft.fftshift(np.fft.fft2(w))), rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------- FD
+def test_fd_vacuum_dispersion(fp):
+    """FD scheme on a tilted plane wave obeys its characteristic equation
+    c+ r^2 - (2 + 4 pi^2 k^2 dz^2) r + c- = 0 (pi = 3.14159 as U.cu:1006)."""
+    N, ps = 32, 0.3
+    s = _vacuum_sim(fp, N, ps, kV=200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 0, 0)
+    sc = s.scalars()
+    dz, lam = np.float32(sc["dzFD"]), sc["wavel"]
+    kx, _ = s.freqs()
+    j = 3
+    k = float(kx[j])
+    assert k < sc["kmax"]
+    pw = np.tile(np.exp(2j * np.pi * j * np.arange(N) / N), (N, 1))
+    p9 = 3.14159
+    a = 2 * p9 * float(dz) / lam
+    cp, cm = 1 + 1j * a, 1 - 1j * a
+    roots = np.roots([cp, -(2 + 4 * p9 ** 2 * k ** 2 * float(dz) ** 2), cm])
+    psi_m, psi0 = 1.0 + 0j, np.exp(-1j * np.pi * float(dz) * lam * k * k)
+    A = np.linalg.solve(np.array([[1, 1], roots]), np.array([psi_m, psi0]))
+    s.init_plane_fd()
+    s.set_wave(pw * psi_m, 1)
+    s.set_wave(pw * psi0, 0)
+    for n in range(1, 6):
+        s.step_fd(n)
+        want = A[0] * roots[0] ** (n + 1) + A[1] * roots[1] ** (n + 1)
+        assert np.max(np.abs(s.wave() - want * pw)) < 1e-10
+
+
+# ---------------------------------------------------------------- adjoint
+def test_oracle_adjoint_matches_central_differences():
+    N, ns, npb = 16, 4, 2
+    s = _vacuum_sim(np.ones((103, 12), np.float32), N, 0.3, kV=300.0)
+    rng = np.random.default_rng(7)
+    V = (rng.random((ns, N, N)) * 40).astype(np.float32)
+    psi0 = rng.standard_normal((npb, N, N)) + 1j * rng.standard_normal((npb, N, N))
+    target = rng.random((npb, N, N))
+    I, _ = s.diff(V, psi0, None)
+    _, g = s.diff(V, psi0, 2 * (I - target))
+    for (i, y, x) in [(0, 1, 2), (1, 5, 5), (3, 15, 0), (2, 8, 11)]:
+        h = 1e-2
+        Vp = V.astype(np.float64).copy()
+        vals = []
+        for sgn in (1, -1):
+            Vq = V.copy()
+            Vq[i, y, x] = np.float32(V[i, y, x] + sgn * h)
+            Iq, _ = s.diff(Vq, psi0, None)
+            vals.append(np.sum((Iq - target) ** 2))
+        hh = float(np.float32(V[i, y, x] + h)) - float(np.float32(V[i, y, x] - h))
+        fd = (vals[0] - vals[1]) / hh
+        assert g[i, y, x] == pytest.approx(fd, rel=1e-5, abs=1e-10)
+
+
+# ---------------------------------------------------------------- loader
+def test_xyz_reference_symbol_map(tmp_path):
+    assert O.ref_znum("Si") == 14 and O.ref_znum("Sr") == 38 and O.ref_znum("O") == 8
+    assert O.ref_znum("Ti") == 1 and O.ref_znum("Au") == 1  # unknown -> H (U.cu:1241)
+    p = tmp_path / "a.xyz"
+    p.write_text("3\n\nSi 0 0 0\nTi 1.5 2 3\nO 4 5 6.25\n")
+    s = O.OracleSim()
+    assert s.load_xyz_refmap(str(p)) == 3
+    assert list(s.extents()) == [6.0, -2.0, 7.0, -2.0, 8.25, -2.0]
+    p.write_text("2\ncomment line here\nSi 0 0 0\nO 1 1 1\n")
+    assert s.load_xyz_refmap(str(p)) == 2
