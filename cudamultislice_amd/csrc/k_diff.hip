@@ -1,0 +1,115 @@
+// k_diff.hip -- adjoint row pass of the differentiable multislice.
+//
+// Forward slice i (row pass, then column pass):
+//   a_i = F_x( t_i . w_i ),  w_i = IF_x(z_{i-1}) (= psi_{i-1}),  t_i = exp(i sigma V_i)
+//   z_i = IF_y( P F_y(a_i) ) / (nx ny)
+// With G the gradient dL/dRe + i dL/dIm carried backwards, the adjoint of the
+// row pass is
+//   G_u = IF_x(G_a),  G_w = conj(t_i) G_u,  G_{z_{i-1}} = F_x(G_w)
+//   dL/dV_i = -sigma Im(conj(G_w) w_i)        (summed over probes)
+// computed here in one pass: per row, every probe's G_a and z_{i-1} are read
+// once, G is written back in place, and the gradient row is reduced over the
+// probes of the workgroup in a fixed order (no atomics).
+#include "msl_diff.hpp"
+#include "msl_fft.hpp"
+
+namespace msl {
+
+template <int N> struct AdjLines { static constexpr int L = (256 / Plan<N>::T) > 0 ? 256 / Plan<N>::T : 1; };
+
+template <int N, bool FIRST>
+__global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs a) {
+  constexpr int T = Plan<N>::T, E = N / T, LINES = AdjLines<N>::L, LS = N + 1;
+  __shared__ float2 lds[LINES * LS];
+  const int tid = threadIdx.x, t = tid % T, ln = tid / T;
+  const int y = blockIdx.x, g = blockIdx.y;
+  const LdsLine L{lds + ln * LS, 1};
+  const float* vr = a.V + (long long)y * N;
+  float2 tr[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) {
+    float sn, cs;
+    sincosf(a.sigma * vr[t + c * T], &sn, &cs);
+    tr[c] = make_float2(cs, sn);
+  }
+  float acc[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) acc[c] = 0.f;
+  for (int q = 0; q < a.per_line; ++q) {
+    const int p = (g * a.per_line + q) * LINES + ln;
+    const bool ok = p < a.nbatch;
+    float2 gu[E], w[E];
+    const long long ro = (long long)p * a.bstride + (long long)y * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      gu[c] = ok ? a.G[ro + t + c * T] : make_float2(0.f, 0.f);
+      w[c] = ok ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
+    }
+    fft_line<N, +1>(gu, t, a.tw, L);             // G_u = IF_x(G_a)
+    if constexpr (!FIRST) fft_line<N, +1>(w, t, a.tw, L);  // w = IF_x(z_{i-1})
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const float2 gw = cmulc(gu[c], tr[c]);     // conj(t) G_u
+      acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
+      gu[c] = gw;
+    }
+    fft_line<N, -1>(gu, t, a.tw, L);             // G_{z_{i-1}} = F_x(G_w)
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < E; ++c) a.Gout[ro + t + c * T] = gu[c];
+    }
+  }
+  // reduce the LINES partial gradient rows in a fixed order
+  __syncthreads();
+  float* red = reinterpret_cast<float*>(lds);
+#pragma unroll
+  for (int c = 0; c < E; ++c) red[ln * N + t + c * T] = acc[c];
+  __syncthreads();
+  if (ln == 0) {
+    float* out = a.partial + (long long)g * a.ny * N + (long long)y * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      float s = 0.f;
+      for (int l = 0; l < LINES; ++l) s += red[l * N + t + c * T];
+      out[t + c * T] = s;
+    }
+  }
+}
+
+template <int N> static hipError_t adj_n(const AdjArgs& a, bool first, hipStream_t s) {
+  constexpr int LINES = AdjLines<N>::L, T = Plan<N>::T;
+  dim3 grid(a.ny, a.ngroups), block(LINES * T);
+  if (first) hipLaunchKernelGGL((k_row_adj<N, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_row_adj<N, false>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+int adj_lines(int nx) {
+  switch (nx) {
+    case 16: return AdjLines<16>::L;
+    case 32: return AdjLines<32>::L;
+    case 64: return AdjLines<64>::L;
+    case 128: return AdjLines<128>::L;
+    case 256: return AdjLines<256>::L;
+    case 512: return AdjLines<512>::L;
+    case 1024: return AdjLines<1024>::L;
+    case 2048: return AdjLines<2048>::L;
+  }
+  return 0;
+}
+
+hipError_t launch_row_adjoint(const AdjArgs& a, bool first, hipStream_t s) {
+  switch (a.nx) {
+    case 16: return adj_n<16>(a, first, s);
+    case 32: return adj_n<32>(a, first, s);
+    case 64: return adj_n<64>(a, first, s);
+    case 128: return adj_n<128>(a, first, s);
+    case 256: return adj_n<256>(a, first, s);
+    case 512: return adj_n<512>(a, first, s);
+    case 1024: return adj_n<1024>(a, first, s);
+    case 2048: return adj_n<2048>(a, first, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace msl

@@ -1,0 +1,606 @@
+// k_physics.hip -- atom binning, potential synthesis, probe, CTF, detector
+// and finite-difference kernels.  Citations are to the reference's
+// "GUI Simulation/UnmanagedGUISimulation.cu" (U.cu).
+#include <cmath>
+
+#include "msl_fft.hpp"
+#include "msl_internal.hpp"
+
+namespace msl {
+
+static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ============================================================== binning
+// atombin2 (U.cu:277-303): bin ids from FP32 sub, div, mul(50), floor --
+// evaluated with IEEE division and no contraction so ids are bit-exact.
+__global__ void k_bin_keys(BinArgs a) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const float t1 = a.x[i] - a.MinX, t2 = a.MaxX - a.MinX;
+  const float t3 = t1 / t2;
+  const float t4 = t3 * 50.0f;
+  const int bidx = (int)floorf(t4);
+  const float u1 = a.y[i] - a.MinY, u2 = a.MaxY - a.MinY;
+  const float u3 = u1 / u2;
+  const float u4 = u3 * 50.0f;
+  const int bidy = (int)floorf(u4);
+  const float w1 = a.MaxZ - a.z[i];
+  const float w2 = w1 / 1.0f;
+  const int zid = (int)floorf(w2);
+  if (bidx < 0 || bidx >= 50 || bidy < 0 || bidy >= 50 || zid < 0 || zid >= a.nslab) {
+    a.key[i] = -1;
+    atomicOr(a.bad, 1);
+    return;
+  }
+  const int key = zid * 2500 + bidy * 50 + bidx;
+  a.key[i] = key;
+  atomicAdd(&a.count[key], 1);
+}
+
+// exclusive scan of count[0..nb] into start[0..nb], three phases
+constexpr int SCAN_B = 256, SCAN_PER = 4, SCAN_CHUNK = SCAN_B * SCAN_PER;
+
+__device__ int block_excl_scan(int v, int* sh, int& total) {
+  // Hillis-Steele over 256 threads
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_B; off <<= 1) {
+    int x = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += x;
+    __syncthreads();
+  }
+  total = sh[SCAN_B - 1];
+  const int incl = sh[tid];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void __launch_bounds__(SCAN_B) k_scan_local(const int* in, int* out, int n, int* sums) {
+  __shared__ int sh[SCAN_B];
+  const long long base = (long long)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER;
+  int v[SCAN_PER], s = 0;
+  for (int k = 0; k < SCAN_PER; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int ex = block_excl_scan(s, sh, total);
+  for (int k = 0; k < SCAN_PER; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(SCAN_B) k_scan_sums(int* sums, int nsums) {
+  __shared__ int sh[SCAN_B];
+  int carry = 0;
+  for (int b0 = 0; b0 < nsums; b0 += SCAN_B) {
+    const int i = b0 + threadIdx.x;
+    const int v = i < nsums ? sums[i] : 0;
+    int total;
+    const int ex = block_excl_scan(v, sh, total);
+    if (i < nsums) sums[i] = ex + carry;
+    carry += total;
+  }
+}
+
+__global__ void k_scan_add(int* out, int n, const int* sums) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / SCAN_CHUNK];
+}
+
+__global__ void k_bin_scatter(BinArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int key = a.key[i];
+  if (key < 0) return;
+  a.perm[a.start[key] + atomicAdd(&a.fill[key], 1)] = i;
+}
+
+// restore input order inside each bin (the reference's push_back order,
+// U.cu:1479-1482), so the CSR is identical on every run
+__global__ void k_bin_sort(BinArgs a, int nb) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int s = a.start[b], e = a.start[b + 1];
+  for (int i = s + 1; i < e; ++i) {
+    const int v = a.perm[i];
+    int j = i - 1;
+    while (j >= s && a.perm[j] > v) {
+      a.perm[j + 1] = a.perm[j];
+      --j;
+    }
+    a.perm[j + 1] = v;
+  }
+}
+
+// positions relative to Minimum*, U.cu:1479-1481
+__global__ void k_bin_gather(BinArgs a, int nb) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.start[nb]) return;  // only slots filled by k_bin_scatter
+  const int i = a.perm[s];
+  if (i < 0 || i >= a.n) return;
+  a.bx[s] = a.x[i] - a.MinX;
+  a.by[s] = a.y[i] - a.MinY;
+  a.bz[s] = a.z[i] - a.MinZ;
+  a.bZ[s] = a.Z[i];
+}
+
+hipError_t launch_bin(const BinArgs& a, hipStream_t s) {
+  const int nb = a.nslab * 2500;
+  hipLaunchKernelGGL(k_bin_keys, dim3(nblk(a.n, 256)), dim3(256), 0, s, a);
+  const int nscan = nb + 1;  // count[nb] == 0, so start[nb] = total
+  const int nchunks = (int)nblk(nscan, SCAN_CHUNK);
+  hipLaunchKernelGGL(k_scan_local, dim3(nchunks), dim3(SCAN_B), 0, s, a.count, a.start, nscan, a.scan_tmp);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_B), 0, s, a.scan_tmp, nchunks);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk(nscan, 256)), dim3(256), 0, s, a.start, nscan, a.scan_tmp);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(nblk(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_bin_sort, dim3(nblk(nb, 256)), dim3(256), 0, s, a, nb);
+  hipLaunchKernelGGL(k_bin_gather, dim3(nblk(a.n, 256)), dim3(256), 0, s, a, nb);
+  return hipGetLastError();
+}
+
+// ============================================================== potential
+// Per-Z constants, host side.  vatom (U.cu:160-193):
+//   V(r) = sum_i A_i exp2(-B_i r) / r + sum_i C_i exp2(-D_i r^2)
+//   A = 150.4121417 a, B = 2 pi sqrt(b) log2 e, C = 266.5157269 c d^-1.5,
+//   D = pi^2 / d log2 e
+// vzatom2 (U.cu:119-152):
+//   Vz(r) = sum_i A'_i K0(B'_i r) + sum_i C'_i exp2(-D_i r^2)
+//   A' = 300.8242834 a, B' = 2 pi sqrt(b), C' = 150.4121417 c / d
+void kirkland_transform(const float* fp, float* zt) {
+  const double pi = 3.141592654, l2e = 1.4426950408889634;
+  for (int z = 0; z < 103; ++z) {
+    const float* p = fp + z * 12;
+    float* o = zt + z * 24;
+    for (int i = 0; i < 3; ++i) {
+      double a = p[2 * i], b = p[2 * i + 1], c = p[6 + 2 * i], d = p[6 + 2 * i + 1];
+      o[2 * i] = (float)(150.4121417 * a);
+      o[2 * i + 1] = (float)(2.0 * pi * std::sqrt(b) * l2e);
+      o[6 + 2 * i] = (float)(266.5157269 * c * std::pow(d, -1.5));
+      o[6 + 2 * i + 1] = (float)(pi * pi / d * l2e);
+      o[12 + 2 * i] = (float)(300.8242834 * a);
+      o[12 + 2 * i + 1] = (float)(2.0 * pi * std::sqrt(b));
+      o[18 + 2 * i] = (float)(150.4121417 * c / d);
+      o[18 + 2 * i + 1] = (float)(pi * pi / d * l2e);
+    }
+  }
+}
+
+__device__ __forceinline__ float vatom_dev(const float* c, float r2) {
+  float r = sqrtf(r2);
+  r = fmaxf(r, 0.25f);
+  const float rr = r * r;
+  const float sl = c[0] * exp2f(-c[1] * r) + c[2] * exp2f(-c[3] * r) + c[4] * exp2f(-c[5] * r);
+  const float sg = c[6] * exp2f(-c[7] * rr) + c[8] * exp2f(-c[9] * rr) + c[10] * exp2f(-c[11] * rr);
+  return sl / r + sg;
+}
+
+// Abramowitz & Stegun 9.8.1-9.8.6 in FP32 (the forms of U.cu:41-111)
+__device__ __forceinline__ float bessi0_dev(float x) {
+  const float ax = fabsf(x);
+  if (ax <= 3.75f) {
+    float t = x / 3.75f;
+    t = t * t;
+    return 1.0f + t * (3.5156229f + t * (3.0899424f + t * (1.2067492f + t * (0.2659732f + t * (0.0360768f + t * 0.0045813f)))));
+  }
+  const float t = 3.75f / ax;
+  const float s = 0.39894228f + t * (0.01328592f + t * (0.00225319f + t * (-0.00157565f + t * (0.00916281f +
+                  t * (-0.02057706f + t * (0.02635537f + t * (-0.01647633f + t * 0.00392377f)))))));
+  return expf(ax) * s / sqrtf(ax);
+}
+
+__device__ __forceinline__ float bessk0_dev(float x) {
+  const float ax = fabsf(x);
+  if (ax > 0.f && ax <= 2.0f) {
+    float x2 = ax * 0.5f;
+    x2 = x2 * x2;
+    const float s = -0.57721566f + x2 * (0.42278420f + x2 * (0.23069756f + x2 * (0.03488590f + x2 * (0.00262698f +
+                    x2 * (0.00010750f + x2 * 0.00000740f)))));
+    return -logf(ax * 0.5f) * bessi0_dev(x) + s;
+  } else if (ax > 2.0f) {
+    const float x2 = 2.0f / ax;
+    const float s = 1.25331414f + x2 * (-0.07832358f + x2 * (0.02189568f + x2 * (-0.01062446f + x2 * (0.00587872f +
+                    x2 * (-0.00251540f + x2 * 0.00053208f)))));
+    return expf(-ax) * s / sqrtf(ax);
+  }
+  return 1.0e20f;
+}
+
+__device__ __forceinline__ float vzatom_dev(const float* c, float r2) {
+  float r = sqrtf(r2);
+  r = fmaxf(r, 0.25f);
+  const float rr = r * r;
+  const float sl = c[12] * bessk0_dev(c[13] * r) + c[14] * bessk0_dev(c[15] * r) + c[16] * bessk0_dev(c[17] * r);
+  const float sg = c[18] * exp2f(-c[19] * rr) + c[20] * exp2f(-c[21] * rr) + c[22] * exp2f(-c[23] * rr);
+  return sl + sg;
+}
+
+// FP32 distance decisions of the reference kernels (U.cu:637-640, 784, 931):
+// pixel position fma(x, ps, offset), squared distance as fma(dy, dy, dx*dx)
+// and fma(dz, dz, rxy).  Written out explicitly, with contraction off, so
+// the r^2 < 5 cutoff is decided the same way on every build.
+__device__ __forceinline__ float pix_pos(int i, float ps, float off) {
+#pragma clang fp contract(off)
+  return fmaf((float)i, ps, off);
+}
+__device__ __forceinline__ float dist2_xy(float px, float py, float ax, float ay) {
+#pragma clang fp contract(off)
+  const float dx = px - ax, dy = py - ay;
+  return fmaf(dy, dy, dx * dx);
+}
+__device__ __forceinline__ float dist2_z(float rxy, float z, float az) {
+#pragma clang fp contract(off)
+  const float dd = z - az;
+  return fmaf(dd, dd, rxy);
+}
+__device__ __forceinline__ float sample_z(float zf, int m, float dz) {
+#pragma clang fp contract(off)
+  return zf - ((float)m * dz) / 20.0f;
+}
+
+constexpr int PT = 16;      // pixel tile edge
+constexpr int ACAP = 1024;  // atoms staged in LDS per chunk
+
+// binnedAtomPKernel3 (U.cu:582-653), binnedAtomPKernelConventional3
+// (U.cu:730-795) and binnedAtomPKernelFD (U.cu:883-946).  One workgroup
+// owns a 16x16 pixel tile of one slice; the atoms of every CSR bin within
+// sqrt(5) A of the tile (slab window per mode) are staged through LDS in
+// chunks, then each thread sums its pixel's contributions.
+__global__ void __launch_bounds__(256) k_potential(PotArgs a) {
+  __shared__ float zt[103 * 24];
+  __shared__ float sx[ACAP], sy[ACAP], sz[ACAP];
+  __shared__ int sZ[ACAP];
+  __shared__ int rs[64], rl[65];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 103 * 24; i += 256) zt[i] = a.ztab[i];
+  const int ix = blockIdx.x * PT + (tid % PT), iy = blockIdx.y * PT + (tid / PT);
+  const int slice = blockIdx.z;
+  float zf, h;
+  {
+#pragma clang fp contract(off)
+    h = a.MaxZ - a.MinZ;
+    zf = (a.mode == 2) ? a.zplane : h - a.dz * (float)(a.first_slice + slice);
+  }
+  int k0, k1;
+  if (a.mode == 1) {
+    k0 = (int)floorf(h - zf);
+    k1 = (int)floorf(h - zf + a.dz) - 1;
+  } else {
+    const int kc = (int)floorf((h - zf) / 1.0f);
+    k0 = kc - a.loadslicesz;
+    k1 = kc + a.loadslicesz;
+  }
+  k0 = max(k0, 0);
+  k1 = min(k1, a.nslab - 1);
+  const float offx = a.mode == 2 ? 0.f : a.offx, offy = a.mode == 2 ? 0.f : a.offy;
+  const float px = pix_pos(ix, a.ps, offx), py = pix_pos(iy, a.ps, offy);
+  // bin window of the tile: every bin that may hold an atom within sqrt(5)
+  const float wx = (a.MaxX - a.MinX) / 50.f, wy = (a.MaxY - a.MinY) / 50.f, reach = 2.2360680f;
+  const float tx0 = offx + (float)(blockIdx.x * PT) * a.ps, tx1 = offx + (float)(blockIdx.x * PT + PT - 1) * a.ps;
+  const float ty0 = offy + (float)(blockIdx.y * PT) * a.ps, ty1 = offy + (float)(blockIdx.y * PT + PT - 1) * a.ps;
+  const int i0 = max(0, (int)floorf((tx0 - reach) / wx) - 1), i1 = min(49, (int)floorf((tx1 + reach) / wx) + 1);
+  const int j0 = max(0, (int)floorf((ty0 - reach) / wy) - 1), j1 = min(49, (int)floorf((ty1 + reach) / wy) + 1);
+  const float h20 = a.dz / 20.0f;
+  float sum = 0.f;
+  __syncthreads();
+  for (int k = k0; k <= k1; ++k) {
+    for (int jb = j0; jb <= j1; jb += 64) {
+      const int nr = min(64, j1 - jb + 1);
+      if (i0 > i1) break;
+      if (tid < nr) {
+        const int bb = k * 2500 + (jb + tid) * 50;
+        rs[tid] = a.start[bb + i0];
+        rl[tid + 1] = a.start[bb + i1 + 1] - a.start[bb + i0];
+      }
+      __syncthreads();
+      if (tid == 0) {
+        rl[0] = 0;
+        for (int r = 0; r < nr; ++r) rl[r + 1] += rl[r];
+      }
+      __syncthreads();
+      const int tot = rl[nr];
+      for (int c0 = 0; c0 < tot; c0 += ACAP) {
+        const int cn = min(ACAP, tot - c0);
+        for (int q = tid; q < cn; q += 256) {
+          const int g = c0 + q;
+          int r = 0;
+          while (rl[r + 1] <= g) ++r;
+          const int src = rs[r] + (g - rl[r]);
+          sx[q] = a.bx[src];
+          sy[q] = a.by[src];
+          sz[q] = a.bz[src];
+          sZ[q] = a.bZ[src];
+        }
+        __syncthreads();
+        for (int l = 0; l < cn; ++l) {
+          const float rxy = dist2_xy(px, py, sx[l], sy[l]);
+          if (rxy >= 5.0f) continue;
+          const float* c = zt + (sZ[l] - 1) * 24;
+          if (a.mode == 0) {
+            const float zr = sz[l];
+            for (int m = 1; m <= 20; ++m) {
+              const float r2 = dist2_z(rxy, sample_z(zf, m, a.dz), zr);
+              if (r2 < 5.0f) sum += h20 * vatom_dev(c, r2);
+            }
+          } else if (a.mode == 1) {
+            sum += vzatom_dev(c, rxy);
+          } else {
+            const float r2 = dist2_z(rxy, zf, sz[l]);
+            if (r2 < 5.0f) sum += vatom_dev(c, r2);
+          }
+        }
+        __syncthreads();
+      }
+      __syncthreads();  // rs / rl are rewritten by the next window row
+    }
+  }
+  if (ix >= a.nx || iy >= a.ny) return;
+  const long long o = (long long)slice * a.out_sstride + (long long)iy * a.nx + ix;
+  if (a.out_kind == 0) {
+    float sn, cs;
+    sincosf(a.sigma * sum, &sn, &cs);
+    reinterpret_cast<float2*>(a.out)[o] = make_float2(cs, sn);
+  } else if (a.out_kind == 1) {
+    reinterpret_cast<float2*>(a.out)[o] = make_float2(sum, 0.f);
+  } else {
+    reinterpret_cast<float*>(a.out)[o] = sum;
+  }
+}
+
+hipError_t launch_potential(const PotArgs& a, hipStream_t s) {
+  if (a.nslices <= 0) return hipSuccess;
+  dim3 grid(nblk(a.nx, PT), nblk(a.ny, PT), a.nslices);
+  hipLaunchKernelGGL(k_potential, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ============================================================== probe
+// STEMinitializingKernel (U.cu:354-383): exp(i(theta - chi)) inside
+// k < apert.  theta = 2 pi (kx posx + ky posy) ps is reduced in FP64 so the
+// ramp stays accurate for positions far from the origin.
+__global__ void k_probe_kspace(ProbeArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  const int b = blockIdx.z;
+  if (x >= a.nx) return;
+  const float k0x = a.kx[x], k0y = a.ky[y];
+  const float k = sqrtf(k0x * k0x + k0y * k0y);
+  float2 v = make_float2(0.f, 0.f);
+  if (k < a.apert) {
+    const int px = a.posx ? a.posx[b] : a.px0, py = a.posy ? a.posy[b] : a.py0;
+    const double k2 = (double)k0x * k0x + (double)k0y * k0y, wl = a.wavel, pi = 3.14159265;
+    const double chi = pi * wl * k2 * ((double)a.Cs * wl * wl * k2 * 0.5 + (double)a.df);
+    const double th = 2.0 * pi * ((double)k0x * px * (double)a.ps + (double)k0y * py * (double)a.ps);
+    double ph = th - chi;
+    ph -= 6.283185307179586 * floor(ph / 6.283185307179586);
+    float sn, cs;
+    sincosf((float)ph, &sn, &cs);
+    v = make_float2(cs, sn);
+  }
+  a.out[(long long)b * a.bstride + (long long)y * a.nx + x] = v;
+}
+
+hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_probe_kspace, dim3(nblk(a.nx, 256), a.ny, a.nprobe), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ============================================================== reductions
+constexpr int RB = 256;
+
+__device__ float block_sum(float v, float* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = RB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const float r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// per wave: partial[b][blk] = sum |w| over a fixed 4096-element chunk
+constexpr int L1_CHUNK = 4096;
+__global__ void __launch_bounds__(RB) k_abs_partial(const float2* w, long long n, long long bstride, float* partial,
+                                                    int nchunk) {
+  __shared__ float sh[RB];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  const float2* p = w + (long long)b * bstride;
+  float acc = 0.f;
+  const long long s0 = (long long)blk * L1_CHUNK;
+  for (int i = threadIdx.x; i < L1_CHUNK; i += RB) {
+    const long long j = s0 + i;
+    if (j < n) acc += sqrtf(p[j].x * p[j].x + p[j].y * p[j].y);
+  }
+  const float r = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[(long long)b * nchunk + blk] = r;
+}
+
+__global__ void __launch_bounds__(RB) k_abs_final(float* partial, int nchunk, float* norm) {
+  __shared__ float sh[RB];
+  const int b = blockIdx.x;
+  float acc = 0.f;
+  for (int i = threadIdx.x; i < nchunk; i += RB) acc += partial[(long long)b * nchunk + i];
+  const float r = block_sum(acc, sh);
+  if (threadIdx.x == 0) norm[b] = r;
+}
+
+__global__ void k_div_by(float2* w, long long n, long long bstride, const float* norm) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= n) return;
+  const float inv = 1.0f / norm[b];
+  float2* p = w + (long long)b * bstride;
+  p[i] = cscale(p[i], inv);
+}
+
+hipError_t launch_l1_normalise(float2* w, long long n, int nbatch, long long bstride, float* scratch, hipStream_t s) {
+  const int nchunk = (int)nblk(n, L1_CHUNK);
+  float* partial = scratch;
+  float* norm = scratch + (long long)nbatch * nchunk;
+  hipLaunchKernelGGL(k_abs_partial, dim3(nchunk, nbatch), dim3(RB), 0, s, w, n, bstride, partial, nchunk);
+  hipLaunchKernelGGL(k_abs_final, dim3(nbatch), dim3(RB), 0, s, partial, nchunk, norm);
+  hipLaunchKernelGGL(k_div_by, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, w, n, bstride, norm);
+  return hipGetLastError();
+}
+
+__global__ void k_fill(float2* w, long long n, float2 v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = v;
+}
+hipError_t launch_fill(float2* w, long long n, float2 v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, w, n, v);
+  return hipGetLastError();
+}
+
+__global__ void k_scale(float2* w, long long n, float f) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = cscale(w[i], f);
+}
+hipError_t launch_scale(float2* w, long long n, float f, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(nblk(n, 256)), dim3(256), 0, s, w, n, f);
+  return hipGetLastError();
+}
+
+__global__ void k_cmul(const float2* a, const float2* b, float2* o, long long n, float f) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = cscale(cmul(a[i], b[i]), f);
+}
+hipError_t launch_cmul(const float2* a, const float2* b, float2* o, long long n, float f, hipStream_t s) {
+  hipLaunchKernelGGL(k_cmul, dim3(nblk(n, 256)), dim3(256), 0, s, a, b, o, n, f);
+  return hipGetLastError();
+}
+
+__global__ void k_reduce_rows(const float* partial, int nbatch, int ntiles, float* out, int acc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatch) return;
+  float s = 0.f;
+  for (int i = 0; i < ntiles; ++i) s += partial[(long long)b * ntiles + i];
+  out[b] = acc ? out[b] + s : s;
+}
+hipError_t launch_reduce_rows(const float* partial, int nbatch, int ntiles, float* out, int acc, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nblk(nbatch, 256)), dim3(256), 0, s, partial, nbatch, ntiles, out, acc);
+  return hipGetLastError();
+}
+
+__global__ void k_abs(const float2* w, float* o, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = sqrtf(w[i].x * w[i].x + w[i].y * w[i].y);
+}
+hipError_t launch_abs(const float2* w, float* o, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_abs, dim3(nblk(n, 256)), dim3(256), 0, s, w, o, n);
+  return hipGetLastError();
+}
+
+__global__ void k_intensity(const float2* w, float* o, long long n, float f) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = (w[i].x * w[i].x + w[i].y * w[i].y) * f;
+}
+hipError_t launch_intensity(const float2* w, float* o, long long n, float f, hipStream_t s) {
+  hipLaunchKernelGGL(k_intensity, dim3(nblk(n, 256)), dim3(256), 0, s, w, o, n, f);
+  return hipGetLastError();
+}
+
+// ============================================================== CTF
+// imagingKernel2, U.cu:1044-1080.  chi and the envelope arguments are
+// evaluated in FP64 (they reach ~1e2 rad at large Cs).
+__global__ void k_ctf(CtfArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, d = blockIdx.z;
+  if (x >= a.nx) return;
+  const long long id = (long long)y * a.nx + x;
+  const float k0x = a.kx[x], k0y = a.ky[y];
+  const float k2f = k0x * k0x + k0y * k0y;
+  const float objap2 = ((a.obj * 0.001f) / a.wavel) * ((a.obj * 0.001f) / a.wavel);
+  float2 out = make_float2(0.f, 0.f);
+  if (k2f < objap2) {
+    const double lam = a.wavel, k2 = (double)k0x * k0x + (double)k0y * k0y, k = sqrt(k2);
+    const double df = a.dfs[d], cs = a.Cs, b = a.beta, dl = a.delta, p1 = 3.141593;
+    const double factor = b * b / (4.0 * lam * lam);
+    const double esa = p1 * k * lam * 2.0 * df + 2.0 * p1 * lam * lam * lam * cs * k2 * k;
+    const double es = exp(-factor * esa * esa);
+    const double ed = exp(-0.25 * dl * dl * p1 * p1 * k2 * k2 * lam * lam);
+    const double phi = atan2((double)k0y, (double)k0x);
+    const double chi = 3.1415926 * lam * k2 *
+                       (0.5 * cs * lam * lam * k2 + df + (double)a.A1 * sin(2.0 * (phi - (double)a.A1phi)) +
+                        2.0 * (double)a.A2 * lam * k * sin(3.0 * (phi - (double)a.A2phi)) / 3.0);
+    double c = chi - 6.283185307179586 * floor(chi / 6.283185307179586);
+    float sn, cs_;
+    sincosf((float)c, &sn, &cs_);
+    const float e = (float)(es * ed) * a.scale;
+    const float2 v = a.in[id];
+    out = make_float2(e * (v.x * cs_ + v.y * sn), e * (-v.x * sn + v.y * cs_));
+  }
+  a.out[(long long)d * a.nx * a.ny + id] = out;
+}
+hipError_t launch_ctf(const CtfArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctf, dim3(nblk(a.nx, 256), a.ny, a.ndf), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// ============================================================== detectors
+// CentreKernel + conjugationKernel + host sum of AddTDSWaves (U.cu:2734-2755)
+__global__ void k_tds_accum(const float2* F, float* tds, int nx, int ny) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= nx) return;
+  const int x2 = x >= nx / 2 ? x - nx / 2 : x + nx / 2;
+  const int y2 = y >= ny / 2 ? y - ny / 2 : y + ny / 2;
+  const float2 v = F[(long long)y2 * nx + x2];
+  tds[(long long)y * nx + x] += sqrtf(v.x * v.x + v.y * v.y);
+}
+hipError_t launch_tds_accum(const float2* F, float* tds, int nx, int ny, hipStream_t s) {
+  hipLaunchKernelGGL(k_tds_accum, dim3(nblk(nx, 256), ny), dim3(256), 0, s, F, tds, nx, ny);
+  return hipGetLastError();
+}
+
+__global__ void k_sum_maps(const float* in, int nb, long long n, float* out, int acc) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; ++b) s += in[(long long)b * n + i];
+  out[i] = acc ? out[i] + s : s;
+}
+hipError_t launch_sum_maps(const float* in, int nb, long long n, float* out, int acc, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_maps, dim3(nblk(n, 256)), dim3(256), 0, s, in, nb, n, out, acc);
+  return hipGetLastError();
+}
+
+// ============================================================== FD
+// finiteDifferenceKernel, U.cu:995-1023 (pi written 3.14159f there)
+__global__ void k_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi, const float2* V,
+                            long long n, float sigma, float wavel, float dz) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float a = 2.f * 3.14159f * dz / wavel;
+  const float den = 1.f + a * a;
+  const float2 rcp = make_float2(1.f / den, -a / den);                     // 1 / (1 + i a)
+  const float2 ratio = make_float2((1.f - a * a) / den, -2.f * a / den);   // (1 - i a) / (1 + i a)
+  const float2 p = psi[i], g = grad[i], m = psim[i];
+  const float vv = dz * dz * 4.f * 3.14159f * sigma * V[i].x / wavel;
+  const float2 A = make_float2(2.f * p.x - dz * dz * g.x - vv * p.x, 2.f * p.y - dz * dz * g.y - vv * p.y);
+  psip[i] = csub(cmul(rcp, A), cmul(ratio, m));
+}
+hipError_t launch_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi, const float2* V,
+                            long long n, float sigma, float wavel, float dz, hipStream_t s) {
+  hipLaunchKernelGGL(k_fd_update, dim3(nblk(n, 256)), dim3(256), 0, s, grad, psim, psip, psi, V, n, sigma, wavel, dz);
+  return hipGetLastError();
+}
+
+// ============================================================== adjoint seed
+// G = dL/dRe + i dL/dIm of the exit wave for a loss of its intensity
+__global__ void k_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  g[i] = cscale(psi[i], 2.f * dLdI[i]);
+}
+hipError_t launch_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n, int nbatch, hipStream_t s) {
+  hipLaunchKernelGGL(k_seed_adjoint, dim3(nblk(n * nbatch, 256)), dim3(256), 0, s, psi, dLdI, g, n * nbatch);
+  return hipGetLastError();
+}
+
+}  // namespace msl

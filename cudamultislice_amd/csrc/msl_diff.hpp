@@ -1,0 +1,24 @@
+// msl_diff.hpp -- launch interface of the adjoint kernels (k_diff.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace msl {
+
+struct AdjArgs {
+  const float2* G;   // gradient w.r.t. the row-pass output a_i, (kx, y) domain
+  const float2* Z;   // z_{i-1} in the (kx, y) domain, or psi0 in real space for the first slice
+  float2* Gout;      // gradient w.r.t. z_{i-1} (may alias G)
+  const float* V;    // potential of slice i, [ny][nx]
+  float sigma;
+  long long bstride; // elements between probes
+  int nx, ny, nbatch;
+  int per_line;      // probes each line processes
+  int ngroups;       // workgroups per row
+  float* partial;    // [ngroups][ny][nx] gradient partial sums
+  const float2* tw;  // twiddles for nx
+};
+
+int adj_lines(int nx);
+hipError_t launch_row_adjoint(const AdjArgs& a, bool first, hipStream_t s);
+
+}  // namespace msl

@@ -1,0 +1,268 @@
+// msl_diff_host.cpp -- differentiable multislice with slice checkpointing.
+//
+// Forward: psi_i = IFFT(P . FFT(exp(i sigma V_i) psi_{i-1})) / (nx ny), the
+// conventional step of U.cu:2085-2124 with the potential given per slice.
+// Backward: adjoint passes in reverse slice order; every K-th intermediate
+// wave is checkpointed by the forward and the waves inside a segment are
+// recomputed from its checkpoint (the last segment is kept from the forward).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/msl.h"
+#include "msl_diff.hpp"
+#include "msl_internal.hpp"
+
+using namespace msl;
+
+namespace {
+
+struct DErr : std::runtime_error {
+  int code;
+  DErr(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+inline void hc(hipError_t e, const char* w) {
+  if (e == hipSuccess) return;
+  throw DErr(e == hipErrorOutOfMemory ? MSL_ERR_OOM : MSL_ERR_HIP, std::string(w) + ": " + hipGetErrorString(e));
+}
+#define HC(x) hc((x), #x)
+
+template <class T> struct Buf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~Buf() { if (p) (void)hipFree(p); }
+  void alloc(size_t c) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = c;
+    if (c) HC(hipMalloc(&p, c * sizeof(T)));
+  }
+};
+
+}  // namespace
+
+struct msl_diff {
+  int device = 0;
+  hipStream_t own = nullptr;
+  std::string err;
+  int nx = 0, ny = 0, ns = 0, B = 0, K = 1, S = 1;
+  float ps = 0, kV = 0, dz = 0, wavel = 0, sigma = 0, kmax = 0, nf = 0;
+  std::vector<float> hkx, hky;
+  Buf<float> kx, ky, partial;
+  Buf<float2> twx, twy, px, py, W, G, exitw, ckpt, seg;
+  const float* V = nullptr;
+  const float2* psi0 = nullptr;
+  bool have_forward = false;
+  int ngroups = 1, per_line = 1;
+  long long np() const { return (long long)nx * ny; }
+};
+
+namespace {
+
+// location of z_j (j >= 1) during the forward / inside a recomputed segment
+float2* zloc(msl_diff* d, int j, int seg_base) {
+  if (j % d->K == 0 && j < d->ns) return d->ckpt.p + (size_t)(j / d->K - 1) * d->B * d->np();
+  if (j > seg_base && j < seg_base + d->K) return d->seg.p + (size_t)(j - seg_base - 1) * d->B * d->np();
+  return d->W.p;
+}
+
+RowArgs rargs(msl_diff* d, const float2* src, float2* dst) {
+  RowArgs a{};
+  a.src = src; a.dst = dst; a.src_bstride = a.dst_bstride = d->np();
+  a.nx = d->nx; a.ny = d->ny; a.nbatch = d->B; a.t_group = d->B; a.t_bstride = 0; a.scale = 1.f;
+  a.tw = d->twx.p; a.sigma = d->sigma;
+  return a;
+}
+ColArgs cargs(msl_diff* d, const float2* src, float2* dst, float scale) {
+  ColArgs a{};
+  a.src = src; a.dst = dst; a.src_bstride = a.dst_bstride = d->np();
+  a.nx = d->nx; a.ny = d->ny; a.nbatch = d->B; a.kx = d->kx.p; a.ky = d->ky.p; a.px = d->px.p; a.py = d->py.p;
+  a.kmax = d->kmax; a.scale = scale; a.tw = d->twy.p;
+  return a;
+}
+
+// slice i (1-based) forward: z_{i-1} (or psi0) -> z_i at dst
+void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, hipStream_t s) {
+  RowArgs r = rargs(d, zprev, d->W.p);
+  r.V = d->V + (size_t)(i - 1) * d->np();
+  HC(launch_row(i == 1 ? ROW_FWD : ROW_INV_MUL_FWD, RM_PHASE, r, s));
+  HC(launch_col(COL_FWD_MUL_INV, CM_PROP, cargs(d, d->W.p, dst, d->nf * d->nf), s));
+}
+
+}  // namespace
+
+extern "C" {
+
+msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps, float kV, float dz, int nslices,
+                           int nprobe, int K) {
+  if (!out) return MSL_ERR_ARG;
+  *out = nullptr;
+  if (!fft_size_supported(nx) || !fft_size_supported(ny)) return MSL_ERR_UNSUPPORTED;
+  if (!(ps > 0) || !(kV > 0) || !(dz > 0) || nslices < 1 || nprobe < 1) return MSL_ERR_ARG;
+  msl_diff* d = new msl_diff;
+  try {
+    d->device = device;
+    HC(hipSetDevice(device));
+    HC(hipStreamCreateWithFlags(&d->own, hipStreamNonBlocking));
+    d->nx = nx; d->ny = ny; d->ps = ps; d->kV = kV; d->dz = dz; d->ns = nslices; d->B = nprobe;
+    d->K = std::max(1, std::min(K <= 0 ? nslices : K, nslices));
+    d->S = (nslices + d->K - 1) / d->K;
+    // FP32 physics of U.cu:1791-1799
+    const float Pi = 3.1415926f, V = kV, e = 1.6e-019f;
+    d->wavel = 6.63e-034f * 3e+008f / std::sqrt((e * V * 1000.f * (2.f * 9.11e-031f * 9e+016f + e * V * 1000.f))) * 1e+010f;
+    d->sigma = (2.f * Pi / (d->wavel * V * 1000.f)) * ((9.11e-031f * 9e+016f + e * V * 1000.f) / (2.f * 9.11e-031f * 9e+016f + e * V * 1000.f));
+    // frequencies and kmax, U.cu:1877-1953
+    const float ksx = ps * (float)nx, ksy = ps * (float)ny;
+    const int imx = (int)std::floor(nx / 2 + 0.5), imy = (int)std::floor(ny / 2 + 0.5);
+    d->hkx.resize(nx); d->hky.resize(ny);
+    for (int i = 1; i <= nx; ++i) d->hkx[i - 1] = ((i - 1) > imx) ? (float)((i - 1) - nx) / ksx : (float)(i - 1) / ksx;
+    for (int i = 1; i <= ny; ++i) d->hky[i - 1] = ((i - 1) > imy) ? (float)((i - 1) - ny) / ksy : (float)(i - 1) / ksy;
+    const float hx = d->hkx[imx - 1] * 1 / 2, hy = d->hky[imy - 1] * 1 / 2;
+    const float kmx = (float)std::pow(hx, 2), kmy = (float)std::pow(hy, 2);
+    d->kmax = std::sqrt(kmy <= kmx ? kmy : kmx);
+    d->nf = 1.0f / std::sqrt((float)(nx * ny));
+    auto tw = [](int n) {
+      std::vector<float2> t(n);
+      for (int k = 0; k < n; ++k) {
+        const double a = -2.0 * M_PI * k / n;
+        t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+      return t;
+    };
+    auto pr = [&](const std::vector<float>& k) {
+      std::vector<float2> t(k.size());
+      for (size_t i = 0; i < k.size(); ++i) {
+        const double a = 3.14159265 * (double)dz * (double)d->wavel * (double)k[i] * (double)k[i];
+        t[i] = make_float2((float)std::cos(a), (float)-std::sin(a));
+      }
+      return t;
+    };
+    auto up = [&](auto& buf, const auto& v) {
+      buf.alloc(v.size());
+      HC(hipMemcpy(buf.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+    };
+    up(d->kx, d->hkx); up(d->ky, d->hky); up(d->twx, tw(nx)); up(d->twy, tw(ny));
+    up(d->px, pr(d->hkx)); up(d->py, pr(d->hky));
+    const size_t wave = (size_t)nprobe * nx * ny;
+    d->W.alloc(wave);
+    d->G.alloc(wave);
+    d->exitw.alloc(wave);
+    if (d->S > 1) d->ckpt.alloc((size_t)(d->S - 1) * wave);
+    if (d->K > 1) d->seg.alloc((size_t)(d->K - 1) * wave);
+    const int lines = adj_lines(nx);
+    d->per_line = std::max(1, std::min(4, (nprobe + lines - 1) / lines));
+    d->ngroups = (nprobe + lines * d->per_line - 1) / (lines * d->per_line);
+    d->partial.alloc((size_t)d->ngroups * nx * ny);
+  } catch (const DErr& e) {
+    const int c = e.code;
+    delete d;
+    return c;
+  }
+  *out = d;
+  return MSL_OK;
+}
+
+void msl_diff_destroy(msl_diff* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->device);
+  if (d->own) {
+    (void)hipStreamSynchronize(d->own);
+    (void)hipStreamDestroy(d->own);
+  }
+  delete d;
+}
+
+const char* msl_diff_last_error(const msl_diff* d) { return d ? d->err.c_str() : "null handle"; }
+
+msl_status msl_diff_info(const msl_diff* d, float* wl, float* sg, float* km, long long* ws) {
+  if (!d) return MSL_ERR_ARG;
+  if (wl) *wl = d->wavel;
+  if (sg) *sg = d->sigma;
+  if (km) *km = d->kmax;
+  if (ws) {
+    const long long w = (long long)d->B * d->np() * 8;
+    *ws = w * (3 + (d->S - 1) + (d->K - 1)) + (long long)d->ngroups * d->np() * 4;
+  }
+  return MSL_OK;
+}
+
+msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float* inten, void* stream) {
+  if (!d || !V || !psi0 || !inten) return MSL_ERR_ARG;
+  try {
+    HC(hipSetDevice(d->device));
+    hipStream_t s = stream ? (hipStream_t)stream : d->own;
+    d->V = V;
+    d->psi0 = (const float2*)psi0;
+    const int last_base = (d->S - 1) * d->K;  // the last segment's waves stay in seg
+    const float2* zprev = d->psi0;
+    for (int i = 1; i <= d->ns; ++i) {
+      float2* dst = zloc(d, i, last_base);
+      fwd_slice(d, i, zprev, dst, s);
+      zprev = dst;
+    }
+    HC(launch_row(ROW_INV, RM_NONE, rargs(d, zprev, d->exitw.p), s));
+    HC(launch_intensity(d->exitw.p, inten, (long long)d->B * d->np(), 1.0f, s));
+    d->have_forward = true;
+  } catch (const DErr& e) {
+    d->err = e.what();
+    return e.code;
+  }
+  d->err.clear();
+  return MSL_OK;
+}
+
+msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream) {
+  if (!d || !dLdI || !gradV) return MSL_ERR_ARG;
+  if (!d->have_forward) {
+    d->err = "forward first";
+    return MSL_ERR_STATE;
+  }
+  try {
+    HC(hipSetDevice(d->device));
+    hipStream_t s = stream ? (hipStream_t)stream : d->own;
+    const long long np = d->np();
+    // G at the exit: 2 dL/dI psi_n, then the adjoint of the final IF_x
+    HC(launch_seed_adjoint(d->exitw.p, dLdI, d->G.p, np, d->B, s));
+    HC(launch_row(ROW_FWD, RM_NONE, rargs(d, d->G.p, d->G.p), s));
+    for (int sg = d->S - 1; sg >= 0; --sg) {
+      const int base = sg * d->K;
+      const int top = std::min(base + d->K, d->ns);
+      if (sg < d->S - 1) {
+        // recompute z_{base+1} .. z_{top-1} from the checkpoint
+        const float2* zprev = base == 0 ? d->psi0 : zloc(d, base, base);
+        for (int j = base + 1; j < top; ++j) {
+          float2* dst = zloc(d, j, base);
+          fwd_slice(d, j, zprev, dst, s);
+          zprev = dst;
+        }
+      }
+      for (int i = top; i > base; --i) {
+        HC(launch_col(COL_FWD_MUL_INV, CM_PROPCONJ, cargs(d, d->G.p, d->G.p, d->nf * d->nf), s));
+        AdjArgs a{};
+        a.G = d->G.p;
+        a.Gout = d->G.p;
+        a.Z = (i == 1) ? d->psi0 : zloc(d, i - 1, base);
+        a.V = d->V + (size_t)(i - 1) * np;
+        a.sigma = d->sigma;
+        a.bstride = np;
+        a.nx = d->nx; a.ny = d->ny; a.nbatch = d->B;
+        a.per_line = d->per_line;
+        a.ngroups = d->ngroups;
+        a.partial = d->partial.p;
+        a.tw = d->twx.p;
+        HC(launch_row_adjoint(a, i == 1, s));
+        HC(launch_sum_maps(d->partial.p, d->ngroups, np, gradV + (size_t)(i - 1) * np, 0, s));
+      }
+    }
+  } catch (const DErr& e) {
+    d->err = e.what();
+    return e.code;
+  }
+  d->err.clear();
+  return MSL_OK;
+}
+
+}  // extern "C"

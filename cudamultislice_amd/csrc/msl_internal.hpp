@@ -1,0 +1,160 @@
+// msl_internal.hpp -- launch interface between the host orchestration
+// (msl_host.cpp) and the HIP kernels (k_pass.hip, k_physics.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace msl {
+
+// ------------------------------------------------------------------ passes
+// Row pass: one transform along x (contiguous) per (wave, row) line.
+enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_INV_MUL_FWD = 2 };
+// pointwise factor applied between the (optional) inverse and forward
+// transforms: none, a stored transmission t, or exp(i sigma V) from a real
+// potential
+enum RowMul { RM_NONE = 0, RM_T = 1, RM_PHASE = 2 };
+
+struct RowArgs {
+  const float2* src;
+  float2* dst;
+  long long src_bstride, dst_bstride;  // elements between consecutive waves
+  int nx, ny, nbatch;
+  const float2* t;        // RM_T: transmission of wave b at t + (b / t_group) * t_bstride
+  const float* V;         // RM_PHASE: potential, same indexing
+  long long t_bstride;
+  int t_group;
+  float sigma;
+  float scale;            // applied with the pointwise factor
+  const int2* shift;      // optional: read source row (y+py)%ny, column (x+px)%nx
+  const float2* tw;       // twiddles for length nx
+};
+
+// Column pass: one transform along y (stride nx) per (wave, column).
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FWD_MUL_INV = 2, COL_FWD_MUL = 3, COL_FWD_DETECT = 4, COL_MUL_INV = 5 };
+// k-space factor: none (scale only), Fresnel propagator with strict
+// |k| < kmax, band limit |k| <= kmax, spectral Laplacian, conjugate propagator
+enum ColMul { CM_SCALE = 0, CM_PROP = 1, CM_BAND = 2, CM_GRAD = 3, CM_PROPCONJ = 4 };
+
+struct ColArgs {
+  const float2* src;
+  float2* dst;
+  long long src_bstride, dst_bstride;
+  int nx, ny, nbatch;
+  const float* kx;        // FP32 spatial frequencies (1/A)
+  const float* ky;
+  const float2* px;       // separable propagator factors exp(-i pi lambda dz k^2)
+  const float2* py;
+  float kmax;
+  float scale;
+  const float2* tw;       // twiddles for length ny
+  // COL_FWD_DETECT: annular sum of |v|^2 over kin <= |k| < kout, times
+  // det_scale, into partial[b * (nx / C) + tile]; optional per-wave
+  // |v|^2 map (fftshifted) into inten
+  float kin, kout, det_scale;
+  float* partial;
+  float* inten;
+};
+
+int col_tile(int ny);  // columns per column-pass workgroup
+
+hipError_t launch_row(int mode, int mul, const RowArgs& a, hipStream_t s);
+hipError_t launch_col(int mode, int mul, const ColArgs& a, hipStream_t s);
+bool fft_size_supported(int n);
+
+// ------------------------------------------------------------------ physics
+struct BinArgs {
+  const float *x, *y, *z;       // input coordinates (already displaced)
+  const int* Z;
+  int n;
+  float MaxX, MinX, MaxY, MinY, MaxZ, MinZ;
+  int nslab;
+  int* key;                     // zid*2500 + bid, or -1 if outside
+  int* count;                   // nslab*2500 (+1) zero-initialised
+  int* start;                   // nslab*2500 + 1
+  int* fill;                    // nslab*2500 scratch
+  int* perm;                    // n
+  float *bx, *by, *bz;          // binned, relative to Minimum*
+  int* bZ;
+  int* bad;                     // set to 1 if any atom is outside the bin grid
+  int* scan_tmp;                // block sums scratch
+};
+hipError_t launch_bin(const BinArgs& a, hipStream_t s);
+
+// Potential synthesis.  mode 0: 20-point z integration of the 3D potential
+// (slabs +-3), 1: projected Bessel/Gaussian potential (slice's own slabs),
+// 2: point 3D potential on the plane (finite-difference mode, no offset).
+// out_kind 0: transmission exp(i sigma V) as float2, 1: real V as float2
+// (V, 0), 2: real V as float.
+struct PotArgs {
+  const float *bx, *by, *bz;
+  const int* bZ;
+  const int* start;
+  int nslab;
+  const float* ztab;            // per-Z transformed Kirkland constants (103 x 12)
+  float MaxX, MinX, MaxY, MinY, MaxZ, MinZ;
+  int nx, ny;
+  float ps, offx, offy;
+  float dz;                     // slice thickness (modes 0, 1)
+  int first_slice, nslices;     // slice i uses z = (MaxZ-MinZ) - dz * i (modes 0, 1)
+  float zplane;                 // mode 2 plane position (single slice)
+  int loadslicesz;
+  float sigma;
+  int mode, out_kind;
+  void* out;                    // [nslices][ny][nx]
+  long long out_sstride;        // elements between slices
+};
+hipError_t launch_potential(const PotArgs& a, hipStream_t s);
+void kirkland_transform(const float* fparams, float* ztab);  // host: 103x12 -> per-Z constants
+
+// Probe in k-space (U.cu:354-383) for nprobe positions: batch b gets
+// (posx[b], posy[b]); or posx == nullptr for the single position (px0, py0).
+struct ProbeArgs {
+  float2* out;
+  long long bstride;
+  int nx, ny, nprobe;
+  const float* kx;
+  const float* ky;
+  const int* posx;
+  const int* posy;
+  int px0, py0;
+  float apert, ps, df, Cs, wavel;
+};
+hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s);
+
+// sum |w| over n elements per wave (deterministic), then w /= sum.
+hipError_t launch_l1_normalise(float2* w, long long n, int nbatch, long long bstride, float* scratch, hipStream_t s);
+hipError_t launch_fill(float2* w, long long n, float2 v, hipStream_t s);
+hipError_t launch_scale(float2* w, long long n, float s, hipStream_t st);
+hipError_t launch_cmul(const float2* a, const float2* b, float2* out, long long n, float s, hipStream_t st);
+hipError_t launch_reduce_rows(const float* partial, int nbatch, int ntiles, float* out, int accumulate, hipStream_t s);
+hipError_t launch_abs(const float2* w, float* out, long long n, hipStream_t s);
+
+// CTF (imagingKernel2, U.cu:1044-1080) on a k-space wave for ndf defocus
+// values: out[d] = E_s E_d exp(-i chi) in[...] inside the aperture.
+struct CtfArgs {
+  const float2* in;
+  float2* out;
+  int nx, ny, ndf;
+  const float* kx;
+  const float* ky;
+  const float* dfs;
+  float Cs, A1, A1phi, A2, A2phi, obj, wavel, beta, delta, scale;
+};
+hipError_t launch_ctf(const CtfArgs& a, hipStream_t s);
+hipError_t launch_intensity(const float2* w, float* out, long long n, float s, hipStream_t st);
+
+// TDS / CBED: tds[y][x] += |F[(y+ny/2)%ny][(x+nx/2)%nx]| (U.cu:2734-2766)
+hipError_t launch_tds_accum(const float2* F, float* tds, int nx, int ny, hipStream_t s);
+// sum over nbatch maps of n floats in fixed order: out (+)= sum_b in[b]
+hipError_t launch_sum_maps(const float* in, int nbatch, long long n, float* out, int accumulate, hipStream_t s);
+
+// Finite-difference update (finiteDifferenceKernel, U.cu:995-1023)
+hipError_t launch_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi,
+                            const float2* V, long long n, float sigma, float wavel, float dz, hipStream_t s);
+
+// differentiable multislice helpers
+hipError_t launch_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n, int nbatch,
+                               hipStream_t s);
+
+}  // namespace msl

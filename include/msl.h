@@ -1,0 +1,196 @@
+/* msl.h -- C-ABI of the MI355X multislice engine (libmsl.so).
+ *
+ * Drop-in boundary for the reference's native engine class
+ * UnmanagedMultisliceSimulation ("GUI Simulation/unmanagedGUISimulation.cuh",
+ * U.cuh:62-199; methods defined in "GUI Simulation/UnmanagedGUISimulation.cu",
+ * U.cu).  Each entry point below names the reference method it replaces.
+ * The C++ facade in UnmanagedMultisliceSimulation.h keeps the reference's
+ * method names and signatures on top of these calls.
+ *
+ * Conventions: plain pointers and sizes; every call returns an msl_status and
+ * records a message readable with msl_last_error().  Results are copied into
+ * caller-owned buffers.  A handle owns its device memory and one HIP stream;
+ * a handle is not thread-safe, distinct handles are independent.  Units as
+ * in the reference: angstrom, kV, mrad (apertures, beta), radians (angles).
+ * Waves are complex64 stored as interleaved (re, im) float pairs, row-major
+ * [resY][resX].  Grid sizes must be powers of two from 16 to 2048.
+ */
+#ifndef MSL_H
+#define MSL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct msl_sim msl_sim;
+typedef int msl_status;
+
+enum {
+  MSL_OK = 0,
+  MSL_ERR_ARG = 1,         /* invalid argument */
+  MSL_ERR_STATE = 2,       /* call out of sequence (e.g. step before init) */
+  MSL_ERR_IO = 3,          /* file missing or malformed */
+  MSL_ERR_HIP = 4,         /* HIP runtime error */
+  MSL_ERR_UNSUPPORTED = 5, /* size or mode not supported */
+  MSL_ERR_OOM = 6,         /* device allocation failed */
+  MSL_ERR_RANGE = 7,       /* atom outside the bin grid / no stable FD step */
+  MSL_ERR_CANCELLED = 8    /* msl_cancel() was called */
+};
+
+/* compatibility flags (msl_set_compat) */
+enum {
+  MSL_COMPAT_REFERENCE_SYMBOLS = 1 /* XYZ symbols outside GetZNum's table map to H (U.cu:1188-1243) */
+};
+
+/* potential modes (msl_set_potential_mode) */
+enum {
+  MSL_POT_INTEGRATED = 0, /* 20-point z integral of the 3D potential (binnedAtomPKernel3, U.cu:582-653) */
+  MSL_POT_PROJECTED = 1   /* Bessel/Gaussian projected potential (binnedAtomPKernelConventional3, U.cu:730-795) */
+};
+
+/* ---- lifetime: constructor / destructor, U.cu:2551-2561 ---- */
+msl_status msl_create(msl_sim** out, int device);
+void msl_destroy(msl_sim* sim);
+const char* msl_last_error(const msl_sim* sim);
+const char* msl_status_string(msl_status st);
+msl_status msl_set_compat(msl_sim* sim, unsigned flags);
+msl_status msl_set_potential_mode(msl_sim* sim, int mode);
+msl_status msl_cancel(msl_sim* sim); /* polled between slices by batched runs */
+
+/* ---- structure: LoadAtomFile, U.cu:1293-1356 ---- */
+msl_status msl_load_atom_file(msl_sim* sim, const char* path);
+/* atoms given directly (Z in 1..103); extents follow LoadAtomFile's rule */
+msl_status msl_set_atoms(msl_sim* sim, int n, const int* Z, const float* x, const float* y, const float* z);
+/* UploadBinnedAtoms, U.cu:1358-1536 */
+msl_status msl_upload_binned_atoms(msl_sim* sim);
+/* UploadBinnedAtomsTDS, U.cu:1538-1716: each call bins the next frozen-phonon
+ * configuration, displaced by seeded Philox4x64-10 Gaussians (sigma per axis)
+ * in place of the reference's unseeded rand() */
+msl_status msl_set_tds(msl_sim* sim, uint64_t seed, float sigma);
+msl_status msl_upload_binned_atoms_tds(msl_sim* sim);
+/* bin with caller-given displacements (any may be NULL) */
+msl_status msl_upload_binned_atoms_displaced(msl_sim* sim, const float* dx, const float* dy, const float* dz);
+/* the displacements msl_upload_binned_atoms_tds uses for `config` */
+msl_status msl_tds_displacements(uint64_t seed, int config, int n, float sigma, float* dx, float* dy, float* dz);
+/* GetParameterisation, U.cu:1718-1747: path NULL or "" selects the embedded table */
+msl_status msl_get_parameterisation(msl_sim* sim, const char* path);
+/* copy the 103x12 table in use */
+msl_status msl_get_fparams(const msl_sim* sim, float* out);
+
+/* ---- microscope and sampling ---- */
+/* ApplyMicroscopeParameters, U.cu:1749-1761 */
+msl_status msl_apply_microscope_parameters(msl_sim* sim, float kV, float df, float Mod2f, float Arg2f, float Cs,
+                                           float beta, float delta, float aperture);
+/* SetCalculationVariables, U.cu:1763-1875 */
+msl_status msl_set_calculation_variables(msl_sim* sim, float ps, float df, int resX, int resY, float sizeX,
+                                         float sizeY, float sizeZ, float offx, float offy);
+/* STEMSetCalculationVariables, U.cu:2567-2670 */
+msl_status msl_stem_set_calculation_variables(msl_sim* sim, float ps, float df, int resX, int resY, float sizeX,
+                                              float sizeY, float sizeZ);
+/* PreCalculateFrequencies, U.cu:1877-1964 */
+msl_status msl_precalculate_frequencies(msl_sim* sim);
+
+/* ---- waves ---- */
+/* InitialiseWavefunctions(), U.cu:1966-2001 (finite-difference start) */
+msl_status msl_initialise_wavefunctions(msl_sim* sim);
+/* InitialiseWavefunctions(float dz), U.cu:2003-2038 (conventional) */
+msl_status msl_initialise_wavefunctions_dz(msl_sim* sim, float dz);
+/* STEMInitialiseWavefunctions, U.cu:2672-2724 */
+msl_status msl_stem_initialise_wavefunctions(msl_sim* sim, int posx, int posy);
+/* conventional slice thickness used by STEM runs (the reference leaves it
+ * at its last TEM value, U.cu:2005) */
+msl_status msl_set_conventional_dz(msl_sim* sim, float dz);
+/* MultisliceStep (finite difference), U.cu:2040-2083 */
+msl_status msl_multislice_step(msl_sim* sim, int iteration);
+/* MultisliceStepConv, U.cu:2085-2124 */
+msl_status msl_multislice_step_conv(msl_sim* sim, int iteration);
+/* slice count as ManagedMultislice::GetSlices computes it (MM.cpp:105-111) */
+msl_status msl_get_slices(const msl_sim* sim, int conventional, int* out);
+
+/* ---- results ---- */
+/* GetExitWave, U.cu:2231-2284: out (2*resX*resY floats) may be NULL; also
+ * sets maxposEW / minposEW */
+msl_status msl_get_exit_wave(msl_sim* sim, float* out);
+/* GetSTEMExitWave, U.cu:2180-2229: fftshifted FFT of the wave */
+msl_status msl_get_stem_exit_wave(msl_sim* sim, float* out);
+/* AllocTDS / AddTDSWaves / GetTDS*, U.cu:2726-2791 */
+msl_status msl_alloc_tds(msl_sim* sim);
+msl_status msl_add_tds_waves(msl_sim* sim);
+msl_status msl_get_tds(msl_sim* sim, float* out);
+/* deterministic part of SimulateImage (U.cu:2322-2343): the CTF image
+ * |IFFT(CTF . FFT(psi))|^2, stored for msl_get_image */
+msl_status msl_simulate_image(msl_sim* sim, float dose);
+msl_status msl_get_image(msl_sim* sim, float* out);
+/* CTF images for several defoci from the current wave: out is ndf*resX*resY */
+msl_status msl_ctf_sweep(msl_sim* sim, int ndf, const float* dfs, float* out);
+/* annular dark-field signal of the current wave: sum of |FFT_unitary(psi)|^2
+ * over inner <= lambda |k| < outer (mrad) */
+msl_status msl_adf(msl_sim* sim, float inner_mrad, float outer_mrad, float* out);
+
+typedef struct {
+  float MaximumX, MinimumX, MaximumY, MinimumY, MaximumZ, MinimumZ; /* U.cuh:127-132 */
+  int numberOfSlices;                                             /* FD slices, U.cuh:133 */
+  float kmax, conventionaldz;                                     /* U.cuh:134-135 */
+  float maxposIm, maxposEW, minposEW, minposIm;                   /* U.cuh:137-140 */
+  float imagecontrast;                                            /* U.cuh:197 */
+  float wavelength, sigma2, dzFD;
+  int natoms, nslab, resX, resY;
+} msl_info;
+msl_status msl_get_info(const msl_sim* sim, msl_info* out);
+
+/* ---- batched STEM + frozen phonons (no counterpart: the reference runs one
+ * probe position, MW.cs:360-399) ---- */
+typedef struct {
+  int nprobe;
+  const int* posx;        /* probe positions in pixels, as STEMInitialiseWavefunctions */
+  const int* posy;
+  int nconfig;            /* frozen-phonon configurations; 0 or 1 with sigma 0 = static */
+  uint64_t seed;          /* Philox seed of the displacements */
+  float sigma;            /* displacement per axis (A) */
+  float dz;               /* slice thickness (A) */
+  int nslices;            /* 0: ceil((MaxZ - MinZ) / dz) */
+  float adf_inner, adf_outer; /* mrad */
+  int batch;              /* probes per launch, 0: automatic */
+  int use_graph;          /* capture the slice loop in a hipGraph */
+  float* adf_out;         /* [nprobe], mean over configurations */
+  float* pacbed_out;      /* optional [resY][resX]: mean |Psi_hat|^2 (unshifted) */
+  float* exit_out;        /* optional [nconfig][nprobe][resY][resX][2] exit waves (small runs) */
+} msl_stem_batch_desc;
+msl_status msl_stem_batched(msl_sim* sim, const msl_stem_batch_desc* desc);
+
+/* time spent in the last msl_stem_batched, split by stage (milliseconds):
+ * [0] binning, [1] potentials, [2] propagation, [3] total */
+msl_status msl_last_timings(const msl_sim* sim, float* out4);
+
+/* ---- differentiable multislice (no counterpart in the reference) ---- */
+typedef struct msl_diff msl_diff;
+/* psi_i = IFFT(P . FFT(exp(i sigma V_i) psi_{i-1})) / (nx ny), P the Fresnel
+ * propagator of U.cu:399-428 for slice thickness dz; V given per slice. */
+msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps, float kV, float dz, int nslices,
+                           int nprobe, int checkpoint_every);
+void msl_diff_destroy(msl_diff* d);
+const char* msl_diff_last_error(const msl_diff* d);
+/* device pointers; stream is a hipStream_t (NULL = the handle's own) */
+msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float* intensity, void* stream);
+msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream);
+msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, float* kmax, long long* workspace_bytes);
+
+/* ---- device-array primitives (used by the torch binding and tests) ---- */
+/* batched 2D FFT of nbatch [ny][nx] complex64 device arrays, in place,
+ * unnormalised; inverse != 0 for exp(+2 pi i) */
+msl_status msl_fft2_device(int nx, int ny, int nbatch, void* data, int inverse, void* stream);
+/* bin / potential on device arrays held by a handle (debug access) */
+msl_status msl_get_binned(const msl_sim* sim, int* start, float* x, float* y, float* z, int* Z);
+msl_status msl_get_bins(const msl_sim* sim, int* keys);
+/* transmission (mode MSL_POT_*) or, with real != 0, the summed potential of
+ * conventional slice `iteration` (z = MaxZ - MinZ - dz*iteration) */
+msl_status msl_potential_slice(msl_sim* sim, int iteration, float dz, int real, float* out);
+msl_status msl_set_wave(msl_sim* sim, int which, const float* in); /* 0 Psi, 1 PsiMinus, 2 PsiPlus */
+msl_status msl_get_wave(msl_sim* sim, int which, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSL_H */

@@ -47,7 +47,7 @@ int stem_batch(Sim<Real>& s, int nprobe, const int* posx, const int* posy, float
   s.conventionaldz = dz;
   std::vector<std::vector<C>> t(nslices);
   for (int i = 1; i <= nslices; ++i) {
-    Real z = Real(s.MaxZ - s.MinZ) - Real(dz) * Real(i);
+    Real z = Real(s.slice_z(dz, i));
     if (potmode == 1) s.potential_proj(z, Real(dz), t[i - 1]);
     else s.potential_conv(z, Real(dz), t[i - 1]);
   }

@@ -440,28 +440,60 @@ template <class Real> struct Sim {
         }
   }
 
+  // FP32 cutoff decisions, as the reference kernels take them in FP32
+  // (U.cu:637-640, 784, 931): pixel position fma(x, ps, off), squared
+  // distances fma(dy, dy, dx*dx) and fma(dz, dz, rxy).  The potential value
+  // itself is evaluated in Real.
+  static float pix_f(int i, float ps, float off) { return std::fma((float)i, ps, off); }
+  static float d2xy_f(float px, float py, float ax, float ay) {
+    float dx = px - ax, dy = py - ay;
+    return std::fma(dy, dy, dx * dx);
+  }
+  static float d2z_f(float rxy, float z, float az) {
+    float dd = z - az;
+    return std::fma(dd, dd, rxy);
+  }
+  static float sample_z_f(float zf, int m, float dz) { return zf - ((float)m * dz) / 20.0f; }
+
   // binnedAtomPKernel3, U.cu:582-653: 20-point z integration of vatom over
   // the slice below z, slabs floor(H-z) +- 3, transmission exp(i sigma V).
   // mode 0 returns the transmission, mode 3 returns the integrated potential
   // (sumz) as the real part.
+  // FP32 slice position of conventional slice `it`: MaxZ - MinZ - dz*it as
+  // the reference computes it in float (U.cu:2097, 2052)
+  float slice_z(float dz, int it) const {
+    float h = MaxZ - MinZ;
+    float d = dz * (float)it;
+    return h - d;
+  }
+  // slab index floor((MaxZ - MinZ - z) / 1), decided in FP32 as in U.cu:602
+  int slab_of(float z) const {
+    float h = MaxZ - MinZ;
+    float d = h - z;
+    return (int)std::floor(d / 1.0f);
+  }
+
   void potential_conv(Real z, Real convdz, std::vector<C>& out, int mode = 0) const {
     out.assign(npix(), C(0, 0));
-    int kc = (int)std::floor((double)(MaxZ - MinZ) - (double)z);
+    int kc = slab_of((float)z);
     int k0 = std::max(0, kc - loadslicesz), k1 = std::min(nSlab - 1, kc + loadslicesz);
     parallel_for(resY, [&](int yb, int ye) {
       for (int y = yb; y < ye; ++y)
         for (int x = 0; x < resX; ++x) {
           Real px = Real(offx) + Real(x) * Real(ps), py = Real(offy) + Real(y) * Real(ps);
+          float pxf = pix_f(x, ps, offx), pyf = pix_f(y, ps, offy);
           Real sumz = 0;
           for_atoms(px, py, k0, k1, [&](int l) {
+            float rxyf = d2xy_f(pxf, pyf, bx[l], by[l]);
+            if (rxyf >= 5.0f) return;
             Real ddx = px - Real(bx[l]), ddy = py - Real(by[l]);
             Real rxy = ddx * ddx + ddy * ddy;
-            if (rxy >= Real(5)) return;
             for (int m = 1; m <= 20; ++m) {
+              if (!(d2z_f(rxyf, sample_z_f((float)z, m, (float)convdz), bz[l]) < 5.0f)) continue;
               Real z2 = z - Real(m) * convdz / Real(20);
               Real ddz = z2 - Real(bz[l]);
               Real r2 = rxy + ddz * ddz;
-              if (r2 < Real(5)) sumz += convdz / Real(20) * vatom<Real>(bZ[l], r2, fparams.data());
+              sumz += convdz / Real(20) * vatom<Real>(bZ[l], r2, fparams.data());
             }
           });
           size_t id = (size_t)y * resX + x;
@@ -477,19 +509,22 @@ template <class Real> struct Sim {
   // only, sum of vzatom2, transmission exp(i sigma V).
   void potential_proj(Real z, Real convdz, std::vector<C>& out, int mode = 0) const {
     out.assign(npix(), C(0, 0));
-    double h = (double)(MaxZ - MinZ);
-    int k0 = (int)std::floor(h - (double)z), k1 = (int)std::floor(h - (double)z + (double)convdz) - 1;
+    float hd = (MaxZ - MinZ) - (float)z;
+    float he = hd + (float)convdz;
+    int k0 = (int)std::floor(hd), k1 = (int)std::floor(he) - 1;
     k0 = std::max(0, k0); k1 = std::min(nSlab - 1, k1);
     parallel_for(resY, [&](int yb, int ye) {
       for (int y = yb; y < ye; ++y)
         for (int x = 0; x < resX; ++x) {
           Real px = Real(offx) + Real(x) * Real(ps), py = Real(offy) + Real(y) * Real(ps);
+          float pxf = pix_f(x, ps, offx), pyf = pix_f(y, ps, offy);
           Real sumz = 0;
           if (k0 <= k1)
             for_atoms(px, py, k0, k1, [&](int l) {
+              if (!(d2xy_f(pxf, pyf, bx[l], by[l]) < 5.0f)) return;
               Real ddx = px - Real(bx[l]), ddy = py - Real(by[l]);
               Real r2 = ddx * ddx + ddy * ddy;
-              if (r2 < Real(5)) sumz += vzatom<Real>(bZ[l], r2, fparams.data());
+              sumz += vzatom<Real>(bZ[l], r2, fparams.data());
             });
           size_t id = (size_t)y * resX + x;
           if (mode == 3) out[id] = C(sumz, 0);
@@ -502,17 +537,19 @@ template <class Real> struct Sim {
   // slabs +-3, no xy offset; real potential.
   void potential_fd(Real z, std::vector<C>& out) const {
     out.assign(npix(), C(0, 0));
-    int kc = (int)std::floor((double)(MaxZ - MinZ) - (double)z);
+    int kc = slab_of((float)z);
     int k0 = std::max(0, kc - loadslicesz), k1 = std::min(nSlab - 1, kc + loadslicesz);
     parallel_for(resY, [&](int yb, int ye) {
       for (int y = yb; y < ye; ++y)
         for (int x = 0; x < resX; ++x) {
           Real px = Real(x) * Real(ps), py = Real(y) * Real(ps);
+          float pxf = pix_f(x, ps, 0.0f), pyf = pix_f(y, ps, 0.0f);
           Real sumz = 0;
           for_atoms(px, py, k0, k1, [&](int l) {
+            if (!(d2z_f(d2xy_f(pxf, pyf, bx[l], by[l]), (float)z, bz[l]) < 5.0f)) return;
             Real ddx = px - Real(bx[l]), ddy = py - Real(by[l]), ddz = z - Real(bz[l]);
             Real r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (r2 < Real(5)) sumz += vatom<Real>(bZ[l], r2, fparams.data());
+            sumz += vatom<Real>(bZ[l], r2, fparams.data());
           });
           out[(size_t)y * resX + x] = C(sumz, 0);
         }
@@ -582,7 +619,7 @@ template <class Real> struct Sim {
 
   // MultisliceStepConv, U.cu:2085-2124.
   void step_conv(int it, int potmode = 0) {
-    Real z = Real(MaxZ - MinZ) - Real(conventionaldz) * Real(it);
+    Real z = Real(slice_z(conventionaldz, it));
     std::vector<C> t;
     if (potmode == 1) potential_proj(z, Real(conventionaldz), t);
     else potential_conv(z, Real(conventionaldz), t);
@@ -607,7 +644,7 @@ template <class Real> struct Sim {
   // BandLimitKernel (1025-1042), gradKernel (948-962) and
   // finiteDifferenceKernel (995-1023).
   void step_fd(int it) {
-    Real z = Real(MaxZ - MinZ) - Real(dzFD) * Real(it);
+    Real z = Real(slice_z(dzFD, it));
     std::vector<C> V;
     potential_fd(z, V);
     fft.run(V.data(), false);

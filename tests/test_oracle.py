@@ -223,8 +223,14 @@ def _tiny_specimen():
     return Z, x, y, z
 
 
+def _f32_fma(a, b, c):
+    # float32 fused multiply-add: the double product of two floats is exact
+    return np.float32(float(a) * float(b) + float(c))
+
+
 def _brute_conv(s, Z, x, y, z, zc, convdz, N, ps, off):
-    """No bins: every atom in the +-3 slab window, 20 z samples, r^2 < 5."""
+    """No bins: every atom in the +-3 slab window, 20 z samples, r^2 < 5
+    (decided in FP32 as the reference kernel decides it)."""
     e = s.extents()
     H = float(e[4] - e[5])
     kc = int(np.floor(H - zc))
@@ -238,15 +244,21 @@ def _brute_conv(s, Z, x, y, z, zc, convdz, N, ps, off):
     for iy in range(N):
         for ix in range(N):
             px, py = off + ix * ps, off + iy * ps
+            pxf, pyf = _f32_fma(ix, ps, off), _f32_fma(iy, ps, off)
             acc = 0.0
             for a in range(len(Z)):
                 if abs(zid[a] - kc) > 3:
                     continue
+                dxf, dyf = np.float32(pxf - np.float32(X[a])), np.float32(pyf - np.float32(Y[a]))
+                rxyf = _f32_fma(dyf, dyf, np.float32(dxf * dxf))
                 for m in range(1, 21):
+                    z2f = np.float32(np.float32(zc) - np.float32(np.float32(m) * np.float32(convdz)) / np.float32(20))
+                    ddf = np.float32(z2f - np.float32(Zr[a]))
+                    if not _f32_fma(ddf, ddf, rxyf) < np.float32(5):
+                        continue
                     z2 = zc - m * convdz / 20.0
                     r2 = (px - X[a]) ** 2 + (py - Y[a]) ** 2 + (z2 - Zr[a]) ** 2
-                    if r2 < 5.0:
-                        acc += convdz / 20.0 * O.vatom(int(Z[a]), r2, fpar)
+                    acc += convdz / 20.0 * O.vatom(int(Z[a]), r2, fpar)
             out[iy, ix] = acc
     return out
 
