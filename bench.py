@@ -1,0 +1,288 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched STEM + frozen-phonon multislice path at 512^2.
+
+Workload (the configuration the headline metric is quoted on): SrTiO3
+10x10x50 cells (25 000 atoms), 512x512 grid over 40 A, 300 kV, 2 A slices
+(99 slices), 64x64 probe raster (4096 probes, 8 px step, 25 mrad), 16
+frozen-phonon configurations with sigma 0.08 A (Philox seed 0x4D534C), ADF
+60-200 mrad, complex64 waves.  One step = one frozen-phonon configuration:
+binning, potentials for every slice and the propagation of all 4096 probes
+through all slices, ending in the ADF reduction.  The default run times all
+16 configurations after one warm-up configuration.
+
+Alongside the headline value the JSON line reports the per-probe launch path
+(the reference API driven one probe at a time, potential rebuilt per slice),
+fwd+bwd/fwd of the differentiable path on 1024^2 x 100 slices x 64 probes,
+the roofline of the dominant kernel and the CPU restatement's throughput on
+a bounded sample.  Diagnostics go to stderr; stdout carries one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup_cfg3(MultisliceSimulation, S, N=512, L=40.0):
+    Z, x, y, z = S.srtio3(10, 10, 50)
+    sim = MultisliceSimulation(0)
+    sim.set_atoms(Z, x, y, z)
+    sim.GetParameterisation()
+    sim.ApplyMicroscopeParameters(300.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 25.0)
+    inf = sim.info()
+    sim.STEMSetCalculationVariables(L / N, 0.0, N, N, inf["MaximumX"] - inf["MinimumX"],
+                                    inf["MaximumY"] - inf["MinimumY"], inf["MaximumZ"] - inf["MinimumZ"])
+    sim.PreCalculateFrequencies()
+    return sim, (Z, x, y, z)
+
+
+def raster(N=512, n=64, step=8):
+    import numpy as np
+    xs = (np.arange(n) * step) % N
+    px, py = np.meshgrid(xs, xs)
+    return px.ravel().astype(np.int32), py.ravel().astype(np.int32)
+
+
+def per_probe_path(sim, px, py, dz, nslices, nprobe):
+    """The reference's call sequence (MW.cs:366-399) one probe at a time:
+    probe, then per slice the potential rebuild and the step, then the
+    detector.  Returns probe*slice/s."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in range(nprobe):
+        sim.STEMInitialiseWavefunctions(int(px[p]), int(py[p]))
+        sim.set_conventional_dz(dz)
+        for it in range(1, nslices + 1):
+            sim.MultisliceStepConv(it)
+        sim.adf(60.0, 200.0)
+    torch.cuda.synchronize()
+    return nprobe * nslices / (time.perf_counter() - t0)
+
+
+def diff_ratio(reps=3):
+    """fwd+bwd / fwd of the differentiable path: 1024^2, 100 slices of 2 A
+    projected potential of amorphous carbon (1.2 g/cm^3, 102.4x102.4x200 A,
+    min distance 1.4 A), 64 probes (8x8 raster, 25 mrad, 300 kV), loss
+    sum (|psi|^2 - I_target)^2 with I_target from a seeded +-2 % perturbed
+    potential, checkpoint every 10 slices."""
+    import numpy as np
+    import torch
+    from cudamultislice_amd import MultisliceSimulation, POT_PROJECTED
+    from cudamultislice_amd import structures as S
+    from cudamultislice_amd.diff import DiffEngine, MultisliceIntensity
+    N, ns, B, dz, ps = 1024, 100, 64, 2.0, 0.1
+    t0 = time.perf_counter()
+    Z, x, y, z = S.amorphous_carbon()
+    log(f"[diff] amorphous carbon: {Z.size} atoms in {time.perf_counter() - t0:.1f} s")
+    sim = MultisliceSimulation(0)
+    sim.set_atoms(Z, x, y, z)
+    sim.GetParameterisation()
+    sim.UploadBinnedAtoms()
+    sim.set_potential_mode(POT_PROJECTED)
+    sim.ApplyMicroscopeParameters(300.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 25.0)
+    inf = sim.info()
+    off = -float(inf["MinimumX"])  # pixel 0 at x = 0 of the specimen
+    sim.SetCalculationVariables(ps, 0.0, N, N, 1, 1, inf["MaximumZ"] - inf["MinimumZ"], off, off)
+    sim.PreCalculateFrequencies()
+    V = np.stack([sim.potential_slice(i, dz, real=True) for i in range(1, ns + 1)])
+    psi0 = np.zeros((B, N, N), np.complex64)
+    sim.STEMSetCalculationVariables(ps, 0.0, N, N, 1, 1, 1)
+    sim.PreCalculateFrequencies()
+    xs = (np.arange(8) * N // 8 + N // 16)
+    for i, (a, b) in enumerate([(a, b) for b in xs for a in xs]):
+        sim.STEMInitialiseWavefunctions(int(a), int(b))
+        psi0[i] = sim.get_wave(1)
+    sim.close()
+    rng = np.random.Generator(np.random.Philox(S.SEED))
+    Vt = torch.from_numpy(V).cuda()
+    Vp = torch.from_numpy((V * (1 + 0.02 * rng.uniform(-1, 1, V.shape))).astype(np.float32)).cuda()
+    p0 = torch.from_numpy(psi0).cuda()
+    eng = DiffEngine(N, N, ps, 300.0, dz, ns, B, checkpoint_every=10)
+    with torch.no_grad():
+        target = eng.forward(Vp, p0).clone()
+
+    def fwd():
+        with torch.no_grad():
+            I = eng.forward(Vt, p0)
+            return ((I - target) ** 2).sum()
+
+    def fwd_bwd():
+        Vr = Vt.detach().requires_grad_(True)
+        I = MultisliceIntensity.apply(Vr, p0, eng)
+        loss = ((I - target) ** 2).sum()
+        loss.backward()
+        return Vr.grad
+
+    fwd(), fwd_bwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd()
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = fwd_bwd()
+    torch.cuda.synchronize()
+    tfb = (time.perf_counter() - t0) / reps
+    assert torch.isfinite(g).all()
+    ws = eng.info()["workspace_bytes"]
+    eng.close()
+    return {"fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3, "ratio": tfb / tf, "grid": N, "slices": ns, "probes": B,
+            "checkpoint_every": 10, "workspace_GiB": ws / 2**30}
+
+
+def cpu_baseline(atoms, px, py, nprobe, nslices, threads):
+    """The CPU restatement (oracle/, templated to float, its own radix-2 FFT)
+    on the same workload, multithreaded over probes; bounded sample."""
+    import numpy as np
+    from cudamultislice_amd.fparams import fparams
+    from oracle import oracle as O
+    Z, x, y, z = atoms
+    o = O.OracleSim(use_float=True)
+    o.set_atoms(Z, x, y, z)
+    o.set_fparams(fparams())
+    o.bin(O.tds_displacements(0x4D534C, 0, Z.size, 0.08))
+    e = o.extents()
+    o.apply_microscope(300.0, obj=25.0)
+    o.set_calc(40.0 / 512, 0.0, 512, 512, 1, 1, float(e[4] - e[5]), stem=True)
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    o.stem_batch(px[:nprobe], py[:nprobe], 2.0, nslices, 60.0, 200.0, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return nprobe * nslices / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16, help="frozen-phonon configurations timed")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--probes", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=0, help="probes per launch (0: all)")
+    ap.add_argument("--per-probe", type=int, default=8, help="probes timed on the per-probe path (0: skip)")
+    ap.add_argument("--cpu-probes", type=int, default=64)
+    ap.add_argument("--cpu-slices", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the probe/configuration "
+              "batch shards across GPUs with one ADF reduction at the end, not built here)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cudamultislice_amd import MultisliceSimulation
+    from cudamultislice_amd import structures as S
+
+    torch.cuda.set_device(0)
+    sim, atoms = setup_cfg3(MultisliceSimulation, S)
+    inf = sim.info()
+    dz = 2.0
+    nslices = int(math.ceil((inf["MaximumZ"] - inf["MinimumZ"]) / dz))
+    px, py = raster()
+    px, py = px[:args.probes], py[:args.probes]
+    nprobe = px.size
+    log(f"[bench] {atoms[0].size} atoms, {nslices} slices, {nprobe} probes, 512^2")
+
+    # warm-up configurations use seeds distinct from the timed ones
+    sim.stem_batched(px, py, dz, nconfig=args.warmup, seed=S.SEED + 1000, sigma=0.08, batch=args.batch)
+    torch.cuda.synchronize()
+    sim.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = sim.stem_batched(px, py, dz, nconfig=args.steps, seed=S.SEED, sigma=0.08, adf=(60.0, 200.0),
+                         batch=args.batch)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ks = sim.kernel_stats()
+    tm = sim.last_timings()
+    sim.set_profiling(False)
+    ps_per_s = args.steps * nprobe * nslices / dt
+    adf = r["adf"]
+    log(f"[bench] batched: {ps_per_s:.4g} probe*slice/s ({dt:.2f} s), stages {tm}, kernels {ks}")
+    log(f"[bench] ADF mean {adf.mean():.4g} min {adf.min():.4g} max {adf.max():.4g}")
+
+    # roofline of the dominant propagation kernel
+    npix = 512 * 512
+    B = nprobe if args.batch <= 0 else min(args.batch, nprobe)
+    kern = max(("row", "column"), key=lambda k: ks[k]["ms_total"])
+    bytes_launch = 16 * B * npix + (8 * npix if kern == "row" else 0)
+    achieved = bytes_launch / (ks[kern]["ms_avg"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": f"{kern} pass (512, batch {B})", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": ks[kern]["ms_avg"],
+            "other_passes": {k: ks[k]["ms_avg"] for k in ("row", "column", "detector", "potential")}}
+
+    per_probe = None
+    if args.per_probe > 0:
+        sim.set_tds(S.SEED, 0.08)
+        sim.UploadBinnedAtomsTDS()
+        v = per_probe_path(sim, px, py, dz, nslices, args.per_probe)
+        # same path with the potential cached: one probe per launch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.stem_batched(px[:args.per_probe], py[:args.per_probe], dz, nconfig=1, seed=S.SEED, sigma=0.08,
+                         batch=1)
+        torch.cuda.synchronize()
+        v_cached = args.per_probe * nslices / (time.perf_counter() - t0 - sim.last_timings()["potential_ms"] * 1e-3
+                                               - sim.last_timings()["binning_ms"] * 1e-3)
+        per_probe = {"value": v, "cached_potential_value": v_cached, "probes_timed": args.per_probe,
+                     "speedup_batched_over_per_probe": ps_per_s / v,
+                     "speedup_batched_over_cached": ps_per_s / v_cached}
+        log(f"[bench] per-probe {v:.4g}, cached-potential per-probe {v_cached:.4g} probe*slice/s")
+    sim.close()
+
+    diff = None
+    if not args.no_diff:
+        diff = diff_ratio()
+        log(f"[bench] diff: {diff}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        v, t = cpu_baseline(atoms, px, py, args.cpu_probes, args.cpu_slices, threads)
+        cpu = {"value": v, "unit": "probe*slice/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_probes} probes x {args.cpu_slices} slices x 1 configuration of the same "
+                         f"workload (potentials for those slices included), float restatement, {t:.1f} s"}
+        log(f"[bench] cpu baseline {v:.4g} probe*slice/s on {threads} threads ({t:.1f} s)")
+
+    out = {
+        "metric": "probe*slice/s at 512^2 FP32 (batched STEM + TDS)",
+        "value": ps_per_s,
+        "unit": "probe*slice/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64 waves)",
+        "data": "synthetic (SrTiO3 supercell, seeded Philox frozen-phonon displacements)",
+        "config": {"workload": "batched STEM + TDS, SrTiO3 10x10x50, 512^2 over 40 A, 300 kV, 2 A slices",
+                   "probes": nprobe, "slices": nslices, "configs_per_step": 1, "batch": B,
+                   "adf_mrad": [60, 200], "step": "one frozen-phonon configuration"},
+        "stage_ms_total": tm,
+        "per_probe_path": per_probe,
+        "fwd_bwd_over_fwd": diff,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

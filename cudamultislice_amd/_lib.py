@@ -66,6 +66,7 @@ _SIGS = {
     "msl_get_info": ([_vp, C.POINTER(msl_info)], _i),
     "msl_stem_batched": ([_vp, C.POINTER(msl_stem_batch_desc)], _i),
     "msl_last_timings": ([_vp, _vp], _i),
+    "msl_set_profiling": ([_vp, _i], _i), "msl_kernel_stats": ([_vp, _vp], _i),
     "msl_diff_create": ([C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _i, _i, _i], _i),
     "msl_diff_destroy": ([_vp], None), "msl_diff_last_error": ([_vp], _cp),
     "msl_diff_forward": ([_vp, _vp, _vp, _vp, _vp], _i),

@@ -187,8 +187,16 @@ struct msl_sim {
   bool have_tds = false, have_image = false;
   float maxposIm = 0, maxposEW = 0, minposEW = 0, minposIm = 0, imagecontrast = 0;
   float timings[4] = {0, 0, 0, 0};
+  // per-kernel HIP-event timing of batched runs: [row ms, row n, col ms,
+  // col n, detect ms, detect n, potential ms, potential n]
+  bool profiling = false;
+  double kstats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<hipEvent_t> events;
 
   size_t npix() const { return (size_t)resX * resY; }
+  ~msl_sim() {
+    for (auto e : events) (void)hipEventDestroy(e);
+  }
 };
 
 namespace {
@@ -485,6 +493,41 @@ float download_scalar(msl_sim* s, const float* d) {
   HIPC(hipStreamSynchronize(s->stream));
   return v;
 }
+
+// Event pairs around launches of one chunk, summed per kernel kind once the
+// chunk has finished.  kind: 0 row pass, 1 column pass, 2 detector pass,
+// 3 potential.
+struct KTimer {
+  msl_sim* s;
+  std::vector<int> kinds;
+  explicit KTimer(msl_sim* s_) : s(s_) {}
+  void begin(int kind) {
+    if (!s->profiling) return;
+    const size_t i = kinds.size() * 2;
+    while (s->events.size() < i + 2) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      s->events.push_back(e);
+    }
+    HIPC(hipEventRecord(s->events[i], s->stream));
+    kinds.push_back(kind);
+  }
+  void end() {
+    if (!s->profiling) return;
+    HIPC(hipEventRecord(s->events[kinds.size() * 2 - 1], s->stream));
+  }
+  void flush() {
+    if (!s->profiling || kinds.empty()) return;
+    HIPC(hipStreamSynchronize(s->stream));
+    for (size_t k = 0; k < kinds.size(); ++k) {
+      float ms = 0;
+      HIPC(hipEventElapsedTime(&ms, s->events[2 * k], s->events[2 * k + 1]));
+      s->kstats[2 * kinds[k]] += ms;
+      s->kstats[2 * kinds[k] + 1] += 1;
+    }
+    kinds.clear();
+  }
+};
 
 // annular |Psi_unitary|^2 of one real-space wave (copied into scratch)
 float adf_of(msl_sim* s, const float2* w, float inner, float outer) {
@@ -1050,6 +1093,19 @@ msl_status msl_get_wave(msl_sim* s, int which, float* out) {
   API_END(s)
 }
 
+msl_status msl_set_profiling(msl_sim* s, int on) {
+  if (!s) return MSL_ERR_ARG;
+  s->profiling = on != 0;
+  std::fill(s->kstats, s->kstats + 8, 0.0);
+  return MSL_OK;
+}
+
+msl_status msl_kernel_stats(const msl_sim* s, double* out8) {
+  if (!s || !out8) return MSL_ERR_ARG;
+  std::copy(s->kstats, s->kstats + 8, out8);
+  return MSL_OK;
+}
+
 msl_status msl_last_timings(const msl_sim* s, float* out4) {
   if (!s || !out4) return MSL_ERR_ARG;
   std::copy(s->timings, s->timings + 4, out4);
@@ -1165,7 +1221,11 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
     }
     HIPC(hipStreamSynchronize(s->stream));
     double a1 = now_ms();
+    KTimer kt(s);
+    kt.begin(3);
     potential(s, s->potmode, 1, nslices, d->dz, 0.f, 0, tall.p, np);
+    kt.end();
+    kt.flush();
     HIPC(hipStreamSynchronize(s->stream));
     double a2 = now_ms();
     tb += a1 - a0;
@@ -1191,11 +1251,15 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
         r.t = tall.p + (size_t)(i - 1) * np;
         r.t_group = nb;
         r.t_bstride = 0;
+        kt.begin(0);
         row(s, ROW_INV_MUL_FWD, RM_T, r);
+        kt.end();
         ColArgs c = col_args(s, W.p, W.p, nb, np);
         if (i < nslices || want_exit) {
           c.scale = s->nf * s->nf;
+          kt.begin(1);
           col(s, COL_FWD_MUL_INV, CM_PROP, c);
+          kt.end();
         } else {
           c.dst = nullptr;
           c.scale = 1.0f;
@@ -1204,9 +1268,12 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
           c.det_scale = s->nf * s->nf;
           c.partial = partial.p;
           c.inten = d->pacbed_out ? inten.p : nullptr;
+          kt.begin(2);
           col(s, COL_FWD_DETECT, CM_PROP, c);
+          kt.end();
         }
       }
+      kt.flush();
       if (want_exit) {
         row(s, ROW_INV, RM_NONE, row_args(s, W.p, W.p, nb, np));
         HIPC(hipMemcpyAsync(d->exit_out + ((size_t)cfg * d->nprobe + p0) * np * 2, W.p, (size_t)nb * np * sizeof(float2),

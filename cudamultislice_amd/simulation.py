@@ -214,6 +214,17 @@ class MultisliceSimulation:
             out["exit"] = ew
         return out
 
+    def set_profiling(self, on: bool = True):
+        self._c(lib().msl_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self):
+        o = np.zeros(8, np.float64)
+        self._c(lib().msl_kernel_stats(self._h, _p(o)))
+        names = ("row", "column", "detector", "potential")
+        return {n: {"ms_total": float(o[2 * i]), "launches": int(o[2 * i + 1]),
+                    "ms_avg": float(o[2 * i] / o[2 * i + 1]) if o[2 * i + 1] else None}
+                for i, n in enumerate(names)}
+
     def last_timings(self):
         o = np.zeros(4, np.float32)
         self._c(lib().msl_last_timings(self._h, _p(o)))

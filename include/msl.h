@@ -163,6 +163,12 @@ msl_status msl_stem_batched(msl_sim* sim, const msl_stem_batch_desc* desc);
 /* time spent in the last msl_stem_batched, split by stage (milliseconds):
  * [0] binning, [1] potentials, [2] propagation, [3] total */
 msl_status msl_last_timings(const msl_sim* sim, float* out4);
+/* per-kernel timing of batched runs with HIP events on the handle's stream
+ * (enabling resets the counters); out8 = {row ms, row launches, column ms,
+ * column launches, detector ms, detector launches, potential ms, potential
+ * launches} summed over the runs since enabling */
+msl_status msl_set_profiling(msl_sim* sim, int on);
+msl_status msl_kernel_stats(const msl_sim* sim, double* out8);
 
 /* ---- differentiable multislice (no counterpart in the reference) ---- */
 typedef struct msl_diff msl_diff;
