@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB copied once by the plain copy kernel (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the probe/configuration "
@@ -188,6 +190,17 @@ def main():
     from cudamultislice_amd import structures as S
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy > 0:
+        import ctypes
+        from cudamultislice_amd._lib import lib
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        assert lib().msl_copy_device(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)) == 0
+        log(f"[bench] copy kernel moved {nbytes} B read + {nbytes} B write")
+        del src, dst
     sim, atoms = setup_cfg3(MultisliceSimulation, S)
     inf = sim.info()
     dz = 2.0

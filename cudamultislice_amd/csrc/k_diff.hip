@@ -23,7 +23,7 @@ __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs
   __shared__ float2 lds[LINES * LS];
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   const int y = blockIdx.x, g = blockIdx.y;
-  const LdsLine L{lds + ln * LS, 1};
+  const LdsLine<0> L{lds + ln * LS, 1};
   const float* vr = a.V + (long long)y * N;
   float2 tr[E];
 #pragma unroll

@@ -1,6 +1,7 @@
 // k_physics.hip -- atom binning, potential synthesis, probe, CTF, detector
 // and finite-difference kernels.  Citations are to the reference's
 // "GUI Simulation/UnmanagedGUISimulation.cu" (U.cu).
+#include <algorithm>
 #include <cmath>
 
 #include "msl_fft.hpp"
@@ -456,6 +457,37 @@ __global__ void k_fill(float2* w, long long n, float2 v) {
 }
 hipError_t launch_fill(float2* w, long long n, float2 v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, s, w, n, v);
+  return hipGetLastError();
+}
+
+__global__ void k_hash_fill(float2* w, long long n, unsigned seed, int unit) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = (unsigned)i * 2654435761u ^ seed;
+  h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+  const float u = (float)(h & 0xffffff) * (1.0f / 16777216.0f), v = (float)(h >> 8) * (1.0f / 16777216.0f);
+  if (unit) {
+    float sn, cs;
+    sincosf(6.2831853f * u, &sn, &cs);
+    w[i] = make_float2(cs, sn);
+  } else {
+    w[i] = make_float2(2.f * u - 1.f, 2.f * v - 1.f);
+  }
+}
+hipError_t launch_hash_fill(float2* w, long long n, unsigned seed, int unit, hipStream_t s) {
+  hipLaunchKernelGGL(k_hash_fill, dim3(nblk(n, 256)), dim3(256), 0, s, w, n, seed, unit);
+  return hipGetLastError();
+}
+
+// plain streaming copy (16-byte vectors), used to calibrate traffic counters
+__global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+hipError_t launch_copy16(const void* src, void* dst, long long bytes, hipStream_t s) {
+  const long long n = bytes / 16;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)std::min<long long>(nblk(n, 256), 65536)), dim3(256), 0, s,
+                     (const float4*)src, (float4*)dst, n);
   return hipGetLastError();
 }
 

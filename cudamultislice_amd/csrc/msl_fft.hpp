@@ -111,11 +111,29 @@ template <> struct Plan<512> { static constexpr int T = 64, R1 = 8, R2 = 8, R3 =
 template <> struct Plan<1024> { static constexpr int T = 64, R1 = 16, R2 = 8, R3 = 8; };
 template <> struct Plan<2048> { static constexpr int T = 128, R1 = 16, R2 = 16, R3 = 8; };
 
-// LDS addressing of one line: element at position p lives at base + p*stride.
-struct LdsLine {
+// Exchange synchronisation: a whole workgroup, or one wavefront when every
+// line lives inside a single wave (LDS traffic of one wave is processed in
+// order; the fences keep the compiler from moving accesses across).
+struct BlockSync {
+  __device__ static __forceinline__ void sync() { __syncthreads(); }
+};
+struct WaveSync {
+  __device__ static __forceinline__ void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
+// LDS addressing of one line: element p lives at base[pad(p) * stride] with
+// pad(p) = p + (p >> SH) (SH = 0: unpadded), which spreads the power-of-two
+// strides of the Stockham scatter over the LDS banks.
+template <int SH> struct LdsLine {
   float2* base;
   int stride;
-  __device__ __forceinline__ float2& at(int p) const { return base[p * stride]; }
+  static __host__ __device__ constexpr int pad(int p) { return SH > 0 ? p + (p >> SH) : p; }
+  static __host__ __device__ constexpr int span(int n) { return pad(n - 1) + 1; }
+  __device__ __forceinline__ float2& at(int p) const { return base[pad(p) * stride]; }
 };
 
 // One Stockham stage of radix R after L points have been combined.  Input:
@@ -123,9 +141,9 @@ struct LdsLine {
 // x[j + m N/R] = v[b + m NB]; outputs go to y[(j/L) L R + (j%L) + m L].
 // If LAST, outputs stay in registers in canonical order; otherwise they are
 // written to LDS, and read back in canonical order after a barrier.
-template <int N, int T, int R, int L, int DIR, bool LAST>
+template <int N, int T, int R, int L, int DIR, bool LAST, class S, class LL>
 __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const float2* __restrict__ tw,
-                                               const LdsLine& lds) {
+                                               const LL& lds) {
   constexpr int E = N / T, NB = E / R;
   static_assert(E % R == 0, "radix must divide the per-thread count");
   float2 out[E];
@@ -159,29 +177,34 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = out[c];
   } else {
-    __syncthreads();
+    S::sync();
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = lds.at(t + c * T);
-    __syncthreads();
+    S::sync();
   }
 }
 
-// Full transform of one line.  tw points at the N-entry table
-// exp(-2 pi i k / N).  All threads of the workgroup must call this together
-// (it contains barriers).
-template <int N, int DIR> __device__ __forceinline__ void fft_line(float2 (&v)[N / Plan<N>::T], int t,
-                                                                   const float2* __restrict__ tw,
-                                                                   const LdsLine& lds) {
-  using P = Plan<N>;
+// Full transform of one line with plan P (T threads, radices R1 R2 R3).
+// tw points at the N-entry table exp(-2 pi i k / N).  Every thread that
+// shares the synchronisation scope must call this together.
+template <class P, int N, int DIR, class S = BlockSync, class LL = LdsLine<0>>
+__device__ __forceinline__ void fft_line(float2 (&v)[N / P::T], int t, const float2* __restrict__ tw, const LL& lds) {
   constexpr int T = P::T;
   if constexpr (P::R3 > 1) {
-    stockham_stage<N, T, P::R1, 1, DIR, false>(v, t, tw, lds);
-    stockham_stage<N, T, P::R2, P::R1, DIR, false>(v, t, tw, lds);
-    stockham_stage<N, T, P::R3, P::R1 * P::R2, DIR, true>(v, t, tw, lds);
+    stockham_stage<N, T, P::R1, 1, DIR, false, S>(v, t, tw, lds);
+    stockham_stage<N, T, P::R2, P::R1, DIR, false, S>(v, t, tw, lds);
+    stockham_stage<N, T, P::R3, P::R1 * P::R2, DIR, true, S>(v, t, tw, lds);
   } else {
-    stockham_stage<N, T, P::R1, 1, DIR, false>(v, t, tw, lds);
-    stockham_stage<N, T, P::R2, P::R1, DIR, true>(v, t, tw, lds);
+    stockham_stage<N, T, P::R1, 1, DIR, false, S>(v, t, tw, lds);
+    stockham_stage<N, T, P::R2, P::R1, DIR, true, S>(v, t, tw, lds);
   }
+}
+
+// back-compatible form: the default plan of length N
+template <int N, int DIR> __device__ __forceinline__ void fft_line(float2 (&v)[N / Plan<N>::T], int t,
+                                                                   const float2* __restrict__ tw,
+                                                                   const LdsLine<0>& lds) {
+  fft_line<Plan<N>, N, DIR, BlockSync, LdsLine<0>>(v, t, tw, lds);
 }
 
 }  // namespace msl

@@ -56,7 +56,17 @@ struct ColArgs {
   float* inten;
 };
 
-int col_tile(int ny);  // columns per column-pass workgroup
+int col_tile(int ny);  // columns per column-pass workgroup (detector layout)
+// pass layout variants per length (kind 0 row, 1 column), see k_pass.hip
+int pass_variant(int kind, int n);
+void set_pass_variant(int kind, int n, int v);
+int num_pass_variants(int kind);
+hipError_t launch_row_v(int mode, int mul, const RowArgs& a, hipStream_t s, int var);
+hipError_t launch_col_v(int mode, int mul, const ColArgs& a, hipStream_t s, int var);
+// deterministic pseudo-random fill: unit != 0 gives exp(i theta) values
+hipError_t launch_hash_fill(float2* w, long long n, unsigned seed, int unit, hipStream_t s);
+// streaming copy of `bytes` (multiple of 16)
+hipError_t launch_copy16(const void* src, void* dst, long long bytes, hipStream_t s);
 
 hipError_t launch_row(int mode, int mul, const RowArgs& a, hipStream_t s);
 hipError_t launch_col(int mode, int mul, const ColArgs& a, hipStream_t s);

@@ -183,6 +183,21 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
 msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream);
 msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, float* kmax, long long* workspace_bytes);
 
+/* ---- pass layouts and autotuning (no counterpart in the reference) ----
+ * kind 0: row pass, 1: column pass.  Variants are alternative LDS layouts and
+ * synchronisation scopes of the same transform; see DESIGN notes in
+ * k_pass.hip.  msl_bench_pass times `reps` launches on an n x n batch of
+ * nbatch waves and, with maxdiff, compares against variant 0;
+ * msl_autotune_passes selects the fastest agreeing variant per pass for n
+ * (out: 3 row + 4 column timings in ms, -1 where rejected). */
+msl_status msl_set_pass_variant(int kind, int n, int variant);
+msl_status msl_get_pass_variant(int kind, int n, int* variant);
+msl_status msl_bench_pass(int kind, int variant, int n, int nbatch, int reps, float* ms, float* maxdiff);
+msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* col_var, float* out);
+/* plain streaming device copy (bytes a multiple of 16); the reference point
+ * for hardware traffic counters */
+msl_status msl_copy_device(void* dst, const void* src, long long bytes, void* stream);
+
 /* ---- device-array primitives (used by the torch binding and tests) ---- */
 /* batched 2D FFT of nbatch [ny][nx] complex64 device arrays, in place,
  * unnormalised; inverse != 0 for exp(+2 pi i) */
