@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="probes per launch (0: all)")
     ap.add_argument("--per-probe", type=int, default=8, help="probes timed on the per-probe path (0: skip)")
     ap.add_argument("--cpu-probes", type=int, default=64)
-    ap.add_argument("--cpu-slices", type=int, default=20)
+    ap.add_argument("--cpu-slices", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -228,11 +228,15 @@ def main():
     log(f"[bench] batched: {ps_per_s:.4g} probe*slice/s ({dt:.2f} s), stages {tm}, kernels {ks}")
     log(f"[bench] ADF mean {adf.mean():.4g} min {adf.min():.4g} max {adf.max():.4g}")
 
-    # roofline of the dominant propagation kernel
+    # roofline of the dominant propagation kernel.  Algorithmic bytes: the
+    # wave batch read + written once per pass (complex64); with band pruning
+    # only the kx columns inside the propagator band (half) move; the row
+    # pass also reads the shared transmission slice once.
     npix = 512 * 512
     B = nprobe if args.batch <= 0 else min(args.batch, nprobe)
     kern = max(("row", "column"), key=lambda k: ks[k]["ms_total"])
-    bytes_launch = 16 * B * npix + (8 * npix if kern == "row" else 0)
+    band = 0.5 if inf["band_tiles"] > 0 else 1.0
+    bytes_launch = int(16 * B * npix * band) + (8 * npix if kern == "row" else 0)
     achieved = bytes_launch / (ks[kern]["ms_avg"] * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": f"{kern} pass (512, batch {B})", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

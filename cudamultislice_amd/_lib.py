@@ -25,7 +25,7 @@ class msl_info(C.Structure):
         ("numberOfSlices", C.c_int), ("kmax", C.c_float), ("conventionaldz", C.c_float),
         ("maxposIm", C.c_float), ("maxposEW", C.c_float), ("minposEW", C.c_float), ("minposIm", C.c_float),
         ("imagecontrast", C.c_float), ("wavelength", C.c_float), ("sigma2", C.c_float), ("dzFD", C.c_float),
-        ("natoms", C.c_int), ("nslab", C.c_int), ("resX", C.c_int), ("resY", C.c_int)]
+        ("natoms", C.c_int), ("nslab", C.c_int), ("resX", C.c_int), ("resY", C.c_int), ("band_tiles", C.c_int)]
 
 
 class msl_stem_batch_desc(C.Structure):

@@ -17,7 +17,12 @@ namespace msl {
 
 template <int N> struct AdjLines { static constexpr int L = (256 / Plan<N>::T) > 0 ? 256 / Plan<N>::T : 1; };
 
-template <int N, bool FIRST>
+// active canonical blocks of a band-limited line (kx < N/4 or >= 3N/4)
+template <int E> __device__ __forceinline__ constexpr bool band_block(int c) {
+  return c < (E / 4 > 0 ? E / 4 : 1) || c >= E - (E / 4 > 0 ? E / 4 : 1);
+}
+
+template <int N, bool FIRST, bool PRUNE>
 __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs a) {
   constexpr int T = Plan<N>::T, E = N / T, LINES = AdjLines<N>::L, LS = N + 1;
   __shared__ float2 lds[LINES * LS];
@@ -42,8 +47,9 @@ __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs
     const long long ro = (long long)p * a.bstride + (long long)y * N;
 #pragma unroll
     for (int c = 0; c < E; ++c) {
-      gu[c] = ok ? a.G[ro + t + c * T] : make_float2(0.f, 0.f);
-      w[c] = ok ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
+      const bool band = !PRUNE || band_block<E>(c);
+      gu[c] = (ok && band) ? a.G[ro + t + c * T] : make_float2(0.f, 0.f);
+      w[c] = (ok && (FIRST || band)) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
     }
     fft_line<N, +1>(gu, t, a.tw, L);             // G_u = IF_x(G_a)
     if constexpr (!FIRST) fft_line<N, +1>(w, t, a.tw, L);  // w = IF_x(z_{i-1})
@@ -56,7 +62,8 @@ __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs
     fft_line<N, -1>(gu, t, a.tw, L);             // G_{z_{i-1}} = F_x(G_w)
     if (ok) {
 #pragma unroll
-      for (int c = 0; c < E; ++c) a.Gout[ro + t + c * T] = gu[c];
+      for (int c = 0; c < E; ++c)
+        if (!PRUNE || band_block<E>(c)) a.Gout[ro + t + c * T] = gu[c];
     }
   }
   // reduce the LINES partial gradient rows in a fixed order
@@ -79,8 +86,13 @@ __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs
 template <int N> static hipError_t adj_n(const AdjArgs& a, bool first, hipStream_t s) {
   constexpr int LINES = AdjLines<N>::L, T = Plan<N>::T;
   dim3 grid(a.ny, a.ngroups), block(LINES * T);
-  if (first) hipLaunchKernelGGL((k_row_adj<N, true>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((k_row_adj<N, false>), grid, block, 0, s, a);
+  if (first) {
+    if (a.prune) hipLaunchKernelGGL((k_row_adj<N, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_row_adj<N, true, false>), grid, block, 0, s, a);
+  } else {
+    if (a.prune) hipLaunchKernelGGL((k_row_adj<N, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_row_adj<N, false, false>), grid, block, 0, s, a);
+  }
   return hipGetLastError();
 }
 

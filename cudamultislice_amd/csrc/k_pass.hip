@@ -30,13 +30,11 @@ static int lg2(int n) {
 // ------------------------------------------------------------------ variants
 // row:    0 workgroup barriers, unpadded LDS
 //         1 wave-local exchanges (lines never straddle a wave), LDS pad 1/16
-//         2 wave-local exchanges, LDS pad 1/8
 // column: 0 [row][col] interleaved LDS, workgroup barriers
 //         1 [col][row] LDS with pad 1/16, workgroup barriers
-//         2 staged: coalesced tile load into [col][row] LDS, then one
-//           wavefront per column with wave-local exchanges (16 columns)
-//         3 staged with 8 columns per workgroup
-static int g_row_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+// (a staged column layout with one wavefront per column and an LDS pad of
+// 1/8 were measured slower or equal on MI355X and were removed)
+static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1};
 static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
@@ -44,14 +42,13 @@ void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 3 : 4; }
+int num_pass_variants(int kind) { return 2; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
 template <int N, int CV> struct ColCfg {
   static constexpr int T = Plan<N>::T;
-  static constexpr bool STAGED = (CV >= 2) && (T == 64);
-  static constexpr int C = STAGED ? (CV == 2 ? (N >= 1024 ? 8 : 16) : (N >= 1024 ? 4 : 8)) : ColTile0<N>::C;
+  static constexpr int C = ColTile0<N>::C;
   static constexpr int SH = (CV == 0) ? 0 : 4;
   // [col][row] line stride in float2: span + padding so consecutive columns
   // start two banks apart (stride = 2 mod 32 float2)
@@ -77,16 +74,35 @@ int col_tile(int ny) {
 
 bool fft_size_supported(int n) { return col_tile(n) > 0; }
 
+// column tiles at each end of kx holding the band x < N/4, x >= 3N/4
+int band_tiles_lo(int nx) {
+  const int C = col_tile(nx);
+  if (C <= 0 || (nx / 4) % C != 0 || nx / 4 < C) return 0;
+  return nx / 4 / C;
+}
+
 // ------------------------------------------------------------------ rows
 template <int N, int RV> struct RowCfg {
   static constexpr int T = Plan<N>::T;
   static constexpr bool WAVE = RV >= 1 && T <= 64;
-  static constexpr int SH = RV == 1 ? 4 : (RV == 2 ? 3 : 0);
+  static constexpr int SH = RV == 1 ? 4 : 0;
   static constexpr int LINES = (256 / T) > 0 ? 256 / T : 1;
   static constexpr int LS = LdsLine<SH>::span(N) + 1;
 };
 
-template <int N, int MODE, int MUL, bool SHIFT, int RV>
+// Band-limited lines: the propagator of U.cu:419 is zero for |k| >= kmax,
+// about half the Nyquist frequency, so between slices only the kx columns
+// x < N/4 and x >= 3N/4 can be non-zero (the host checks this on the FP32
+// k-grid before enabling it).  With PRUNE_IN the row pass loads only those
+// canonical blocks (the rest is known zero); with PRUNE_OUT it stores only
+// them (the column pass never reads the others).
+enum { PRUNE_IN = 1, PRUNE_OUT = 2 };
+template <int E> struct BandBlocks {
+  static constexpr int LO = E / 4 > 0 ? E / 4 : 1;
+  static __host__ __device__ constexpr bool active(int c) { return c < LO || c >= E - LO; }
+};
+
+template <int N, int MODE, int MUL, int FL, int RV>
 __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * Plan<N>::T)) k_row(RowArgs a) {
   using Cfg = RowCfg<N, RV>;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
@@ -99,21 +115,13 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * Plan<N>::T)) k_row(Row
   const int b = ok ? (int)(line / a.ny) : 0;
   const int y = ok ? (int)(line - (long long)b * a.ny) : 0;
   float2 v[E];
-  if (ok) {
-    const float2* src = a.src + (long long)b * a.src_bstride;
-    if constexpr (SHIFT) {
-      const int2 sh = a.shift[b];
-      const float2* row = src + (long long)((y + sh.y) & (a.ny - 1)) * N;
+  {
+    const float2* row = a.src + (long long)b * a.src_bstride + (long long)y * N;
 #pragma unroll
-      for (int c = 0; c < E; ++c) v[c] = row[(t + c * T + sh.x) & (N - 1)];
-    } else {
-      const float2* row = src + (long long)y * N;
-#pragma unroll
-      for (int c = 0; c < E; ++c) v[c] = row[t + c * T];
+    for (int c = 0; c < E; ++c) {
+      const bool load = ok && (!(FL & PRUNE_IN) || BandBlocks<E>::active(c));
+      v[c] = load ? row[t + c * T] : make_float2(0.f, 0.f);
     }
-  } else {
-#pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = make_float2(0.f, 0.f);
   }
   const LL L{lds + ln * LS, 1};
   if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<Plan<N>, N, +1, S>(v, t, a.tw, L);
@@ -143,7 +151,8 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * Plan<N>::T)) k_row(Row
   if (ok) {
     float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
-    for (int c = 0; c < E; ++c) row[t + c * T] = v[c];
+    for (int c = 0; c < E; ++c)
+      if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];
   }
 }
 
@@ -208,7 +217,10 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
   constexpr int T = Plan<N>::T, E = N / T, C = Cfg::C;
   __shared__ float2 lds[Cfg::LDS];
   const int col = threadIdx.x % C, t = threadIdx.x / C;
-  const int tile = blockIdx.x, b = blockIdx.y;
+  // band-limited runs launch only the tiles at both ends of kx
+  const int tile = (a.tiles_lo > 0 && (int)blockIdx.x >= a.tiles_lo) ? (int)blockIdx.x + a.nx / C - 2 * a.tiles_lo
+                                                                      : (int)blockIdx.x;
+  const int b = blockIdx.y;
   const int x = tile * C + col;
   const long long nx = a.nx;
   float2 v[E];
@@ -241,56 +253,13 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) a.partial[(long long)b * (a.nx / C) + tile] = red[0] * a.det_scale;
+    if (threadIdx.x == 0) a.partial[(long long)b * gridDim.x + blockIdx.x] = red[0] * a.det_scale;
     if (a.dst == nullptr) return;
   }
   {
     float2* dst = a.dst + (long long)b * a.dst_bstride + x;
 #pragma unroll
     for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
-  }
-}
-
-// variants 2, 3 (T == 64): the tile moves between HBM and LDS with
-// coalesced C-column rows; in between, wavefront w transforms column w on
-// its own with wave-local exchanges, so a pass has two workgroup barriers.
-template <int N, int MODE, int MUL, int CV>
-__global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col_staged(ColArgs a) {
-  using Cfg = ColCfg<N, CV>;
-  static_assert(Cfg::STAGED, "staged layout needs one wavefront per column");
-  constexpr int T = 64, E = N / T, C = Cfg::C, LSC = Cfg::LSC, RPT = (C * T) / C;  // rows per load step
-  using LL = LdsLine<Cfg::SH>;
-  __shared__ float2 lds[Cfg::LDS];
-  const int tid = threadIdx.x;
-  const int tile = blockIdx.x, b = blockIdx.y;
-  const long long nx = a.nx;
-  // coalesced load: thread (lc, r0) moves rows r0 + c RPT of column lc
-  {
-    const int lc = tid % C, r0 = tid / C;
-    const float2* src = a.src + (long long)b * a.src_bstride + tile * C + lc;
-    float2 tmp[N / RPT];
-#pragma unroll
-    for (int c = 0; c < N / RPT; ++c) tmp[c] = src[(long long)(r0 + c * RPT) * nx];
-#pragma unroll
-    for (int c = 0; c < N / RPT; ++c) lds[lc * LSC + LL::pad(r0 + c * RPT)] = tmp[c];
-  }
-  __syncthreads();
-  const int w = tid / T, t = tid % T;
-  const LL L{lds + w * LSC, 1};
-  float2 v[E];
-#pragma unroll
-  for (int c = 0; c < E; ++c) v[c] = L.at(t + c * T);
-  WaveSync::sync();
-  col_body<N, MODE, MUL, WaveSync>(v, t, tile * C + w, a, L);
-  WaveSync::sync();
-#pragma unroll
-  for (int c = 0; c < E; ++c) L.at(t + c * T) = v[c];
-  __syncthreads();
-  {
-    const int lc = tid % C, r0 = tid / C;
-    float2* dst = a.dst + (long long)b * a.dst_bstride + tile * C + lc;
-#pragma unroll
-    for (int c = 0; c < N / RPT; ++c) dst[(long long)(r0 + c * RPT) * nx] = lds[lc * LSC + LL::pad(r0 + c * RPT)];
   }
 }
 
@@ -301,14 +270,23 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   const long long nlines = (long long)a.nbatch * a.ny;
   if (nlines <= 0) return hipSuccess;
   dim3 grid((unsigned)((nlines + LINES - 1) / LINES)), block(LINES * T);
-  const bool sh = a.shift != nullptr;
-#define MSL_ROW(M, U, SH) hipLaunchKernelGGL((k_row<N, M, U, SH, RV>), grid, block, 0, s, a)
-  if (mode == ROW_FWD && mul == RM_NONE) { if (sh) MSL_ROW(ROW_FWD, RM_NONE, true); else MSL_ROW(ROW_FWD, RM_NONE, false); }
-  else if (mode == ROW_FWD && mul == RM_T) { if (sh) MSL_ROW(ROW_FWD, RM_T, true); else MSL_ROW(ROW_FWD, RM_T, false); }
-  else if (mode == ROW_FWD && mul == RM_PHASE) { if (sh) MSL_ROW(ROW_FWD, RM_PHASE, true); else MSL_ROW(ROW_FWD, RM_PHASE, false); }
-  else if (mode == ROW_INV && mul == RM_NONE && !sh) MSL_ROW(ROW_INV, RM_NONE, false);
-  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && !sh) MSL_ROW(ROW_INV_MUL_FWD, RM_T, false);
-  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && !sh) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, false);
+  const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0);
+#define MSL_ROW(M, U, F) hipLaunchKernelGGL((k_row<N, M, U, F, RV>), grid, block, 0, s, a)
+  if (mode == ROW_FWD && mul == RM_NONE && fl == 0) MSL_ROW(ROW_FWD, RM_NONE, 0);
+  else if (mode == ROW_FWD && mul == RM_NONE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_NONE, PRUNE_OUT);
+  else if (mode == ROW_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_FWD, RM_T, 0);
+  else if (mode == ROW_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_T, PRUNE_OUT);
+  else if (mode == ROW_FWD && mul == RM_PHASE && fl == 0) MSL_ROW(ROW_FWD, RM_PHASE, 0);
+  else if (mode == ROW_FWD && mul == RM_PHASE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_PHASE, PRUNE_OUT);
+  else if (mode == ROW_INV && mul == RM_NONE && fl == 0) MSL_ROW(ROW_INV, RM_NONE, 0);
+  else if (mode == ROW_INV && mul == RM_NONE && fl == PRUNE_IN) MSL_ROW(ROW_INV, RM_NONE, PRUNE_IN);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_INV_MUL_FWD, RM_T, 0);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_ROW(ROW_INV_MUL_FWD, RM_T, PRUNE_OUT);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT))
+    MSL_ROW(ROW_INV_MUL_FWD, RM_T, PRUNE_IN | PRUNE_OUT);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == 0) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, 0);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == (PRUNE_IN | PRUNE_OUT))
+    MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, PRUNE_IN | PRUNE_OUT);
   else return hipErrorInvalidValue;
 #undef MSL_ROW
   return hipGetLastError();
@@ -316,11 +294,8 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
 
 template <int N>
 static hipError_t row_n(int mode, int mul, const RowArgs& a, hipStream_t s, int var) {
-  switch (var) {
-    case 1: return row_nv<N, 1>(mode, mul, a, s);
-    case 2: return row_nv<N, 2>(mode, mul, a, s);
-    default: return row_nv<N, 0>(mode, mul, a, s);
-  }
+  if (var == 1) return row_nv<N, 1>(mode, mul, a, s);
+  return row_nv<N, 0>(mode, mul, a, s);
 }
 
 template <int N, int CV>
@@ -329,22 +304,9 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
   constexpr int C = Cfg::C;
   if (a.nbatch <= 0) return hipSuccess;
   if (a.nx % C != 0) return hipErrorInvalidValue;
-  dim3 grid(a.nx / C, a.nbatch), block(Cfg::THREADS);
-  if constexpr (Cfg::STAGED) {
-#define MSL_COLS(M, U) hipLaunchKernelGGL((k_col_staged<N, M, U, CV>), grid, block, 0, s, a)
-    if (mode == COL_FWD) MSL_COLS(COL_FWD, CM_SCALE);
-    else if (mode == COL_INV) MSL_COLS(COL_INV, CM_SCALE);
-    else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLS(COL_FWD_MUL_INV, CM_PROP);
-    else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ);
-    else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLS(COL_FWD_MUL_INV, CM_BAND);
-    else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLS(COL_FWD_MUL_INV, CM_GRAD);
-    else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL_INV, CM_SCALE);
-    else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COLS(COL_FWD_MUL, CM_PROP);
-    else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL, CM_SCALE);
-    else if (mode == COL_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_MUL_INV, CM_SCALE);
-    else return hipErrorInvalidValue;
-#undef MSL_COLS
-  } else {
+  if (a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / C) return hipErrorInvalidValue;
+  dim3 grid(a.tiles_lo > 0 ? 2 * a.tiles_lo : a.nx / C, a.nbatch), block(Cfg::THREADS);
+  {
 #define MSL_COL(M, U) hipLaunchKernelGGL((k_col<N, M, U, CV>), grid, block, 0, s, a)
     if (mode == COL_FWD) MSL_COL(COL_FWD, CM_SCALE);
     else if (mode == COL_INV) MSL_COL(COL_INV, CM_SCALE);
@@ -368,10 +330,6 @@ template <int N>
 static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
   // the detector pass reduces over the workgroup in the interleaved layout
   if (mode == COL_FWD_DETECT) return col_nv<N, 0>(mode, mul, a, s);
-  if constexpr (Plan<N>::T == 64) {
-    if (var == 2) return col_nv<N, 2>(mode, mul, a, s);
-    if (var == 3) return col_nv<N, 3>(mode, mul, a, s);
-  }
   if (var == 1) return col_nv<N, 1>(mode, mul, a, s);
   return col_nv<N, 0>(mode, mul, a, s);
 }

@@ -16,6 +16,7 @@ struct AdjArgs {
   int ngroups;       // workgroups per row
   float* partial;    // [ngroups][ny][nx] gradient partial sums
   const float2* tw;  // twiddles for nx
+  int prune;         // band-limited G / z (only x < N/4, x >= 3N/4 non-zero)
 };
 
 int adj_lines(int nx);

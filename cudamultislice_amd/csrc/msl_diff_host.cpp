@@ -57,6 +57,7 @@ struct msl_diff {
   const float2* psi0 = nullptr;
   bool have_forward = false;
   int ngroups = 1, per_line = 1;
+  int prune_tiles = 0;  // band pruning (see the propagation passes)
   long long np() const { return (long long)nx * ny; }
 };
 
@@ -69,8 +70,10 @@ float2* zloc(msl_diff* d, int j, int seg_base) {
   return d->W.p;
 }
 
-RowArgs rargs(msl_diff* d, const float2* src, float2* dst) {
+RowArgs rargs(msl_diff* d, const float2* src, float2* dst, bool pin = false, bool pout = false) {
   RowArgs a{};
+  a.prune_in = pin && d->prune_tiles > 0;
+  a.prune_out = pout && d->prune_tiles > 0;
   a.src = src; a.dst = dst; a.src_bstride = a.dst_bstride = d->np();
   a.nx = d->nx; a.ny = d->ny; a.nbatch = d->B; a.t_group = d->B; a.t_bstride = 0; a.scale = 1.f;
   a.tw = d->twx.p; a.sigma = d->sigma;
@@ -81,12 +84,13 @@ ColArgs cargs(msl_diff* d, const float2* src, float2* dst, float scale) {
   a.src = src; a.dst = dst; a.src_bstride = a.dst_bstride = d->np();
   a.nx = d->nx; a.ny = d->ny; a.nbatch = d->B; a.kx = d->kx.p; a.ky = d->ky.p; a.px = d->px.p; a.py = d->py.p;
   a.kmax = d->kmax; a.scale = scale; a.tw = d->twy.p;
+  a.tiles_lo = d->prune_tiles;
   return a;
 }
 
 // slice i (1-based) forward: z_{i-1} (or psi0) -> z_i at dst
 void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, hipStream_t s) {
-  RowArgs r = rargs(d, zprev, d->W.p);
+  RowArgs r = rargs(d, zprev, d->W.p, i > 1, true);
   r.V = d->V + (size_t)(i - 1) * d->np();
   HC(launch_row(i == 1 ? ROW_FWD : ROW_INV_MUL_FWD, RM_PHASE, r, s));
   HC(launch_col(COL_FWD_MUL_INV, CM_PROP, cargs(d, d->W.p, dst, d->nf * d->nf), s));
@@ -124,6 +128,9 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     const float kmx = (float)std::pow(hx, 2), kmy = (float)std::pow(hy, 2);
     d->kmax = std::sqrt(kmy <= kmx ? kmy : kmx);
     d->nf = 1.0f / std::sqrt((float)(nx * ny));
+    d->prune_tiles = band_tiles_lo(nx);
+    for (int x = nx / 4; x < 3 * nx / 4 && d->prune_tiles > 0; ++x)
+      if (std::sqrt(d->hkx[x] * d->hkx[x] + 0.0f) < d->kmax) d->prune_tiles = 0;
     auto tw = [](int n) {
       std::vector<float2> t(n);
       for (int k = 0; k < n; ++k) {
@@ -152,6 +159,10 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     d->exitw.alloc(wave);
     if (d->S > 1) d->ckpt.alloc((size_t)(d->S - 1) * wave);
     if (d->K > 1) d->seg.alloc((size_t)(d->K - 1) * wave);
+    // columns outside the band are never written by pruned passes: keep them
+    // finite from the start
+    for (Buf<float2>* b : {&d->W, &d->G, &d->exitw, &d->ckpt, &d->seg})
+      if (b->p) HC(hipMemset(b->p, 0, b->n * sizeof(float2)));
     const int lines = adj_lines(nx);
     d->per_line = std::max(1, std::min(4, (nprobe + lines - 1) / lines));
     d->ngroups = (nprobe + lines * d->per_line - 1) / (lines * d->per_line);
@@ -203,7 +214,7 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
       fwd_slice(d, i, zprev, dst, s);
       zprev = dst;
     }
-    HC(launch_row(ROW_INV, RM_NONE, rargs(d, zprev, d->exitw.p), s));
+    HC(launch_row(ROW_INV, RM_NONE, rargs(d, zprev, d->exitw.p, true, false), s));
     HC(launch_intensity(d->exitw.p, inten, (long long)d->B * d->np(), 1.0f, s));
     d->have_forward = true;
   } catch (const DErr& e) {
@@ -226,7 +237,7 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
     const long long np = d->np();
     // G at the exit: 2 dL/dI psi_n, then the adjoint of the final IF_x
     HC(launch_seed_adjoint(d->exitw.p, dLdI, d->G.p, np, d->B, s));
-    HC(launch_row(ROW_FWD, RM_NONE, rargs(d, d->G.p, d->G.p), s));
+    HC(launch_row(ROW_FWD, RM_NONE, rargs(d, d->G.p, d->G.p, false, true), s));
     for (int sg = d->S - 1; sg >= 0; --sg) {
       const int base = sg * d->K;
       const int top = std::min(base + d->K, d->ns);
@@ -253,6 +264,7 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
         a.ngroups = d->ngroups;
         a.partial = d->partial.p;
         a.tw = d->twx.p;
+        a.prune = d->prune_tiles > 0;
         HC(launch_row_adjoint(a, i == 1, s));
         HC(launch_sum_maps(d->partial.p, d->ngroups, np, gradV + (size_t)(i - 1) * np, 0, s));
       }

@@ -174,6 +174,7 @@ struct msl_sim {
   int numberOfSlices = 0, loadblocksx = 0, loadblocksy = 0, loadslicesz = 0;
   bool have_calc = false, have_freqs = false;
   float kmax = 0, nf = 0;
+  int prune_tiles = 0;  // > 0: columns N/4 <= x < 3N/4 are outside the propagator band
   std::vector<float> hkx, hky;
   DBuf<float> kx, ky;
   DBuf<float2> twx, twy, px, py;
@@ -424,6 +425,11 @@ void precalc(msl_sim* s) {
   const float kmx = (float)std::pow(hx, 2), kmy = (float)std::pow(hy, 2);
   s->kmax = std::sqrt((kmy <= kmx) ? kmy : kmx);
   s->nf = 1.0f / std::sqrt((float)(rx * ry));
+  // band pruning is exact when every column N/4 <= x < 3N/4 fails the
+  // propagator's |k| < kmax test for any ky (then so does the band limit's)
+  s->prune_tiles = band_tiles_lo(rx);
+  for (int x = rx / 4; x < 3 * rx / 4 && s->prune_tiles > 0; ++x)
+    if (std::sqrt(s->hkx[x] * s->hkx[x] + 0.0f) < s->kmax) s->prune_tiles = 0;
   s->kx.ensure(rx);
   s->ky.ensure(ry);
   HIPC(hipMemcpyAsync(s->kx.p, s->hkx.data(), rx * sizeof(float), hipMemcpyHostToDevice, s->stream));
@@ -837,13 +843,18 @@ msl_status msl_multislice_step_conv(msl_sim* s, int it) {
   potential(s, s->potmode, it, 1, s->conventionaldz, 0.f, 0, s->V.p, bs);
   ensure_prop(s, s->conventionaldz);
   const float2* src = (it == 1) ? s->PsiMinus.p : s->Psi.p;
+  const bool prune = s->prune_tiles > 0;
   RowArgs r = row_args(s, src, s->PsiPlus.p, 1, bs);
   r.t = s->V.p;
+  r.prune_out = prune;
   row(s, ROW_FWD, RM_T, r);
   ColArgs c = col_args(s, s->PsiPlus.p, s->PsiPlus.p, 1, bs);
   c.scale = s->nf * s->nf;
+  c.tiles_lo = s->prune_tiles;
   col(s, COL_FWD_MUL_INV, CM_PROP, c);
-  row(s, ROW_INV, RM_NONE, row_args(s, s->PsiPlus.p, s->Psi.p, 1, bs));
+  RowArgs ri = row_args(s, s->PsiPlus.p, s->Psi.p, 1, bs);
+  ri.prune_in = prune;
+  row(s, ROW_INV, RM_NONE, ri);
   API_END(s)
 }
 
@@ -1041,6 +1052,7 @@ msl_status msl_get_info(const msl_sim* s, msl_info* o) {
   o->maxposIm = s->maxposIm; o->maxposEW = s->maxposEW; o->minposEW = s->minposEW; o->minposIm = s->minposIm;
   o->imagecontrast = s->imagecontrast; o->wavelength = s->wavel; o->sigma2 = s->sigma2; o->dzFD = s->dzFD;
   o->natoms = (int)s->x.size(); o->nslab = s->nslab; o->resX = s->resX; o->resY = s->resY;
+  o->band_tiles = s->prune_tiles;
   return MSL_OK;
 }
 
@@ -1246,15 +1258,20 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
       ci.scale = 1.0f / norm;
       col(s, COL_INV, CM_SCALE, ci);
       const bool want_exit = d->exit_out != nullptr;
+      const bool prune = s->prune_tiles > 0;
+      int det_tiles = ntiles;
       for (int i = 1; i <= nslices; ++i) {
         RowArgs r = row_args(s, W.p, W.p, nb, np);
         r.t = tall.p + (size_t)(i - 1) * np;
         r.t_group = nb;
         r.t_bstride = 0;
+        r.prune_in = prune && i > 1;  // the probe itself is not band-limited
+        r.prune_out = prune;
         kt.begin(0);
         row(s, ROW_INV_MUL_FWD, RM_T, r);
         kt.end();
         ColArgs c = col_args(s, W.p, W.p, nb, np);
+        c.tiles_lo = s->prune_tiles;
         if (i < nslices || want_exit) {
           c.scale = s->nf * s->nf;
           kt.begin(1);
@@ -1268,6 +1285,9 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
           c.det_scale = s->nf * s->nf;
           c.partial = partial.p;
           c.inten = d->pacbed_out ? inten.p : nullptr;
+          // the |Psi|^2 map needs every column written
+          if (d->pacbed_out) c.tiles_lo = 0;
+          det_tiles = c.tiles_lo > 0 ? 2 * c.tiles_lo : ntiles;
           kt.begin(2);
           col(s, COL_FWD_DETECT, CM_PROP, c);
           kt.end();
@@ -1275,7 +1295,9 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
       }
       kt.flush();
       if (want_exit) {
-        row(s, ROW_INV, RM_NONE, row_args(s, W.p, W.p, nb, np));
+        RowArgs ri = row_args(s, W.p, W.p, nb, np);
+        ri.prune_in = prune;
+        row(s, ROW_INV, RM_NONE, ri);
         HIPC(hipMemcpyAsync(d->exit_out + ((size_t)cfg * d->nprobe + p0) * np * 2, W.p, (size_t)nb * np * sizeof(float2),
                             hipMemcpyDeviceToHost, s->stream));
         // detector on the stored exit waves
@@ -1287,7 +1309,7 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
           h2d(s, adf.p + p0 + q, &cur, sizeof(float));
         }
       } else {
-        HIPC(launch_reduce_rows(partial.p, nb, ntiles, adf.p + p0, 1, s->stream));
+        HIPC(launch_reduce_rows(partial.p, nb, det_tiles, adf.p + p0, 1, s->stream));
         if (d->pacbed_out) HIPC(launch_sum_maps(inten.p, nb, np, pacbed.p, 1, s->stream));
       }
     }

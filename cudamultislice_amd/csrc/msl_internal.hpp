@@ -26,7 +26,7 @@ struct RowArgs {
   int t_group;
   float sigma;
   float scale;            // applied with the pointwise factor
-  const int2* shift;      // optional: read source row (y+py)%ny, column (x+px)%nx
+  int prune_in, prune_out; // band-limited input / output (only x < N/4, x >= 3N/4)
   const float2* tw;       // twiddles for length nx
 };
 
@@ -54,9 +54,13 @@ struct ColArgs {
   float kin, kout, det_scale;
   float* partial;
   float* inten;
+  // > 0: launch only the tiles_lo column tiles at each end of kx (band-limited
+  // data); COL_FWD_DETECT partials are then indexed by launched tile
+  int tiles_lo;
 };
 
 int col_tile(int ny);  // columns per column-pass workgroup (detector layout)
+int band_tiles_lo(int nx);  // column tiles per end covering x < N/4, x >= 3N/4 (0: no pruning)
 // pass layout variants per length (kind 0 row, 1 column), see k_pass.hip
 int pass_variant(int kind, int n);
 void set_pass_variant(int kind, int n, int v);

@@ -137,6 +137,7 @@ typedef struct {
   float imagecontrast;                                            /* U.cuh:197 */
   float wavelength, sigma2, dzFD;
   int natoms, nslab, resX, resY;
+  int band_tiles;  /* > 0: propagation skips the kx columns outside the propagator band (N/4 <= x < 3N/4) */
 } msl_info;
 msl_status msl_get_info(const msl_sim* sim, msl_info* out);
 
