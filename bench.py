@@ -143,6 +143,36 @@ def diff_ratio(reps=3):
             "checkpoint_every": 10, "workspace_GiB": ws / 2**30}
 
 
+def serving_mix(njobs):
+    """The uneven serving mix: njobs jobs with grids in {256, 512, 1024,
+    2048}^2 (0.1 A pixels), 20-200 slices of 2 A or finite-difference steps,
+    half conventional / half finite differences, half plane wave / half STEM
+    with 16-256 probes (25 mrad, 300 kV, ADF 60-200 mrad), 1k-50k atoms with
+    Z in 6..79 at least 1.4 A apart (capped at 0.05 atoms per A^3)."""
+    import numpy as np
+    from cudamultislice_amd.serve import FD, serve, serving_mix as mix
+    t0 = time.perf_counter()
+    jobs = mix(njobs)
+    log(f"[serve] {njobs} jobs generated in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    st = serve(jobs, kV=300.0, ps=0.1, dz=2.0, aperture=25.0, adf=(60.0, 200.0))
+    wall = time.perf_counter() - t0
+    for j in jobs:
+        r = j.result.get("adf", j.result.get("exit"))
+        assert r is not None and np.all(np.isfinite(r))
+    px_slices = sum(max(1, j.nprobe) * j.nslices * j.n * j.n for j in jobs)
+    by = {}
+    for j in jobs:
+        k = f"{j.n}^2 {'fd' if j.method == FD else 'conv'} {'stem' if j.nprobe else 'plane'}"
+        by[k] = by.get(k, 0) + 1
+    log(f"[serve] {st}")
+    return {"jobs": njobs, "wall_s": wall, "jobs_per_s": njobs / wall, "waves": int(st["waves"]),
+            "wave_slices": int(st["wave_slices"]), "pixel_slices_per_s": px_slices / wall,
+            "wave_slices_per_s_512_equivalent": px_slices / (512 * 512) / wall,
+            "engine": st, "classes": by,
+            "atoms": int(sum(j.Z.size for j in jobs)), "probes": int(sum(j.nprobe for j in jobs))}
+
+
 def cpu_baseline(atoms, px, py, nprobe, nslices, threads):
     """The CPU restatement (oracle/, templated to float, its own radix-2 FFT)
     on the same workload, multithreaded over probes; bounded sample."""
@@ -176,6 +206,10 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="time only the differentiable path (fwd, fwd+bwd) and print its line")
+    ap.add_argument("--serve", type=int, default=0, metavar="JOBS",
+                    help="time only the uneven serving mix of JOBS jobs and print its line")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB copied once by the plain copy kernel (traffic-counter reference)")
     args = ap.parse_args()
@@ -190,6 +224,12 @@ def main():
     from cudamultislice_amd import structures as S
 
     torch.cuda.set_device(0)
+    if args.diff_only:
+        print(json.dumps({"fwd_bwd_over_fwd": diff_ratio()}), flush=True)
+        return
+    if args.serve > 0:
+        print(json.dumps({"serving_mix": serving_mix(args.serve)}), flush=True)
+        return
     if args.calibrate_copy > 0:
         import ctypes
         from cudamultislice_amd._lib import lib

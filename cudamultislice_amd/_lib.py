@@ -35,6 +35,19 @@ class msl_stem_batch_desc(C.Structure):
                 ("adf_out", C.c_void_p), ("pacbed_out", C.c_void_p), ("exit_out", C.c_void_p)]
 
 
+class msl_job(C.Structure):
+    _fields_ = [("n", C.c_int), ("method", C.c_int), ("nslices", C.c_int), ("natoms", C.c_int),
+                ("Z", C.c_void_p), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+                ("nprobe", C.c_int), ("posx", C.c_void_p), ("posy", C.c_void_p),
+                ("exit_out", C.c_void_p), ("adf_out", C.c_void_p)]
+
+
+class msl_serve_params(C.Structure):
+    _fields_ = [("kV", C.c_float), ("ps", C.c_float), ("dz", C.c_float), ("aperture", C.c_float),
+                ("adf_inner", C.c_float), ("adf_outer", C.c_float), ("potential_mode", C.c_int),
+                ("budget_bytes", C.c_longlong)]
+
+
 _vp, _i, _f, _cp = C.c_void_p, C.c_int, C.c_float, C.c_char_p
 _SIGS = {
     "msl_create": ([C.POINTER(_vp), _i], _i), "msl_destroy": ([_vp], None),
@@ -81,6 +94,9 @@ _SIGS = {
     "msl_get_bins": ([_vp, _vp], _i),
     "msl_potential_slice": ([_vp, _i, _f, _i, _vp], _i),
     "msl_set_wave": ([_vp, _i, _vp], _i), "msl_get_wave": ([_vp, _i, _vp], _i),
+    "msl_server_create": ([C.POINTER(_vp), _i], _i), "msl_server_destroy": ([_vp], None),
+    "msl_server_last_error": ([_vp], _cp),
+    "msl_server_run": ([_vp, C.POINTER(msl_serve_params), C.POINTER(msl_job), _i, _vp], _i),
 }
 
 _lib = None

@@ -7,28 +7,45 @@
 // row pass is
 //   G_u = IF_x(G_a),  G_w = conj(t_i) G_u,  G_{z_{i-1}} = F_x(G_w)
 //   dL/dV_i = -sigma Im(conj(G_w) w_i)        (summed over probes)
-// computed here in one pass: per row, every probe's G_a and z_{i-1} are read
-// once, G is written back in place, and the gradient row is reduced over the
-// probes of the workgroup in a fixed order (no atomics).
+// computed here in one pass: per row, every probe's G_a and w_i (or z_{i-1})
+// are read once, G is written back in place, and the gradient row is reduced
+// over the probes of the workgroup in a fixed order (no atomics).
 #include "msl_diff.hpp"
 #include "msl_fft.hpp"
 
+#include <type_traits>
+
 namespace msl {
 
-template <int N> struct AdjLines { static constexpr int L = (256 / Plan<N>::T) > 0 ? 256 / Plan<N>::T : 1; };
+// One line per T threads; lines never straddle a wavefront when T <= 64, so
+// the FFT exchanges synchronise the wave only (padded LDS, as the row pass).
+template <int N> struct AdjCfg {
+  static constexpr int T = Plan<N>::T;
+  static constexpr int L = (256 / T) > 0 ? 256 / T : 1;
+  static constexpr bool WAVE = T <= 64;
+  static constexpr int SH = WAVE ? 4 : 0;
+  static constexpr int LS = LdsLine<SH>::span(N) + 1;
+};
 
 // active canonical blocks of a band-limited line (kx < N/4 or >= 3N/4)
 template <int E> __device__ __forceinline__ constexpr bool band_block(int c) {
   return c < (E / 4 > 0 ? E / 4 : 1) || c >= E - (E / 4 > 0 ? E / 4 : 1);
 }
 
-template <int N, bool FIRST, bool PRUNE>
-__global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs a) {
-  constexpr int T = Plan<N>::T, E = N / T, LINES = AdjLines<N>::L, LS = N + 1;
+// REAL_Z: Z holds w_i = psi_{i-1} in real space (stored by the forward row
+// pass, or psi0), so only G is transformed; otherwise Z holds z_{i-1} in the
+// (kx, y) domain and w_i = IF_x(z_{i-1}) is recomputed here.
+template <int N, bool REAL_Z, bool PRUNE>
+__global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a) {
+  using Cfg = AdjCfg<N>;
+  using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
+  using LL = LdsLine<Cfg::SH>;
+  constexpr int T = Plan<N>::T, E = N / T, LINES = Cfg::L, LS = Cfg::LS;
+  static_assert(LS * 2 >= N, "the reduction reuses the line buffers");
   __shared__ float2 lds[LINES * LS];
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   const int y = blockIdx.x, g = blockIdx.y;
-  const LdsLine<0> L{lds + ln * LS, 1};
+  const LL L{lds + ln * LS, 1};
   const float* vr = a.V + (long long)y * N;
   float2 tr[E];
 #pragma unroll
@@ -49,17 +66,17 @@ __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs
     for (int c = 0; c < E; ++c) {
       const bool band = !PRUNE || band_block<E>(c);
       gu[c] = (ok && band) ? a.G[ro + t + c * T] : make_float2(0.f, 0.f);
-      w[c] = (ok && (FIRST || band)) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
+      w[c] = (ok && (REAL_Z || band)) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
     }
-    fft_line<N, +1>(gu, t, a.tw, L);             // G_u = IF_x(G_a)
-    if constexpr (!FIRST) fft_line<N, +1>(w, t, a.tw, L);  // w = IF_x(z_{i-1})
+    fft_line<Plan<N>, N, +1, S>(gu, t, a.tw, L);              // G_u = IF_x(G_a)
+    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S>(w, t, a.tw, L);  // w = IF_x(z_{i-1})
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const float2 gw = cmulc(gu[c], tr[c]);     // conj(t) G_u
       acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
       gu[c] = gw;
     }
-    fft_line<N, -1>(gu, t, a.tw, L);             // G_{z_{i-1}} = F_x(G_w)
+    fft_line<Plan<N>, N, -1, S>(gu, t, a.tw, L);              // G_{z_{i-1}} = F_x(G_w)
     if (ok) {
 #pragma unroll
       for (int c = 0; c < E; ++c)
@@ -83,10 +100,10 @@ __global__ void __launch_bounds__(AdjLines<N>::L * Plan<N>::T) k_row_adj(AdjArgs
   }
 }
 
-template <int N> static hipError_t adj_n(const AdjArgs& a, bool first, hipStream_t s) {
-  constexpr int LINES = AdjLines<N>::L, T = Plan<N>::T;
+template <int N> static hipError_t adj_n(const AdjArgs& a, bool real_z, hipStream_t s) {
+  constexpr int LINES = AdjCfg<N>::L, T = Plan<N>::T;
   dim3 grid(a.ny, a.ngroups), block(LINES * T);
-  if (first) {
+  if (real_z) {
     if (a.prune) hipLaunchKernelGGL((k_row_adj<N, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_row_adj<N, true, false>), grid, block, 0, s, a);
   } else {
@@ -98,28 +115,28 @@ template <int N> static hipError_t adj_n(const AdjArgs& a, bool first, hipStream
 
 int adj_lines(int nx) {
   switch (nx) {
-    case 16: return AdjLines<16>::L;
-    case 32: return AdjLines<32>::L;
-    case 64: return AdjLines<64>::L;
-    case 128: return AdjLines<128>::L;
-    case 256: return AdjLines<256>::L;
-    case 512: return AdjLines<512>::L;
-    case 1024: return AdjLines<1024>::L;
-    case 2048: return AdjLines<2048>::L;
+    case 16: return AdjCfg<16>::L;
+    case 32: return AdjCfg<32>::L;
+    case 64: return AdjCfg<64>::L;
+    case 128: return AdjCfg<128>::L;
+    case 256: return AdjCfg<256>::L;
+    case 512: return AdjCfg<512>::L;
+    case 1024: return AdjCfg<1024>::L;
+    case 2048: return AdjCfg<2048>::L;
   }
   return 0;
 }
 
-hipError_t launch_row_adjoint(const AdjArgs& a, bool first, hipStream_t s) {
+hipError_t launch_row_adjoint(const AdjArgs& a, bool real_z, hipStream_t s) {
   switch (a.nx) {
-    case 16: return adj_n<16>(a, first, s);
-    case 32: return adj_n<32>(a, first, s);
-    case 64: return adj_n<64>(a, first, s);
-    case 128: return adj_n<128>(a, first, s);
-    case 256: return adj_n<256>(a, first, s);
-    case 512: return adj_n<512>(a, first, s);
-    case 1024: return adj_n<1024>(a, first, s);
-    case 2048: return adj_n<2048>(a, first, s);
+    case 16: return adj_n<16>(a, real_z, s);
+    case 32: return adj_n<32>(a, real_z, s);
+    case 64: return adj_n<64>(a, real_z, s);
+    case 128: return adj_n<128>(a, real_z, s);
+    case 256: return adj_n<256>(a, real_z, s);
+    case 512: return adj_n<512>(a, real_z, s);
+    case 1024: return adj_n<1024>(a, real_z, s);
+    case 2048: return adj_n<2048>(a, real_z, s);
   }
   return hipErrorInvalidValue;
 }

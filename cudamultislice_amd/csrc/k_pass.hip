@@ -30,11 +30,18 @@ static int lg2(int n) {
 // ------------------------------------------------------------------ variants
 // row:    0 workgroup barriers, unpadded LDS
 //         1 wave-local exchanges (lines never straddle a wave), LDS pad 1/16
+//         2 as 1 with the alternative plan (AltPlan: other radices and
+//           threads per line)
 // column: 0 [row][col] interleaved LDS, workgroup barriers
 //         1 [col][row] LDS with pad 1/16, workgroup barriers
+//         2 as 0 with the alternative plan
+//         3 as 0 with half as many columns per workgroup (more workgroups
+//           per CU to overlap the barriers)
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
-static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1};
+// defaults from msl_autotune_passes on MI355X (band-pruned passes, 2 GiB of
+// waves): rows 512 and 1024 take the alternative plan (1.27x and 1.02x)
+static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 1};
 static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
@@ -42,19 +49,21 @@ void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return 2; }
+int num_pass_variants(int kind) { return kind == 0 ? 3 : 4; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
 template <int N, int CV> struct ColCfg {
-  static constexpr int T = Plan<N>::T;
-  static constexpr int C = ColTile0<N>::C;
-  static constexpr int SH = (CV == 0) ? 0 : 4;
+  using P = typename std::conditional<CV == 2, AltPlan<N>, Plan<N>>::type;
+  static constexpr bool INTERLEAVED = CV != 1;
+  static constexpr int T = P::T;
+  static constexpr int C = CV == 3 ? ColTile0<N>::C / 2 : ColTile0<N>::C;
+  static constexpr int SH = INTERLEAVED ? 0 : 4;
   // [col][row] line stride in float2: span + padding so consecutive columns
   // start two banks apart (stride = 2 mod 32 float2)
   static constexpr int SPAN = LdsLine<SH>::span(N);
   static constexpr int LSC = SPAN + ((2 - SPAN % 32) + 32) % 32;
-  static constexpr int LDS = (CV == 0) ? N * C : C * LSC;
+  static constexpr int LDS = INTERLEAVED ? N * C : C * LSC;
   static constexpr int THREADS = C * T;
 };
 
@@ -83,9 +92,10 @@ int band_tiles_lo(int nx) {
 
 // ------------------------------------------------------------------ rows
 template <int N, int RV> struct RowCfg {
-  static constexpr int T = Plan<N>::T;
+  using P = typename std::conditional<RV == 2, AltPlan<N>, Plan<N>>::type;
+  static constexpr int T = P::T;
   static constexpr bool WAVE = RV >= 1 && T <= 64;
-  static constexpr int SH = RV == 1 ? 4 : 0;
+  static constexpr int SH = RV >= 1 ? 4 : 0;
   static constexpr int LINES = (256 / T) > 0 ? 256 / T : 1;
   static constexpr int LS = LdsLine<SH>::span(N) + 1;
 };
@@ -96,18 +106,21 @@ template <int N, int RV> struct RowCfg {
 // k-grid before enabling it).  With PRUNE_IN the row pass loads only those
 // canonical blocks (the rest is known zero); with PRUNE_OUT it stores only
 // them (the column pass never reads the others).
-enum { PRUNE_IN = 1, PRUNE_OUT = 2 };
+// STORE_AUX: the pass also writes the real-space wave of ROW_INV_MUL_FWD
+// (the adjoint reuses it instead of transforming again).
+enum { PRUNE_IN = 1, PRUNE_OUT = 2, STORE_AUX = 4 };
 template <int E> struct BandBlocks {
   static constexpr int LO = E / 4 > 0 ? E / 4 : 1;
   static __host__ __device__ constexpr bool active(int c) { return c < LO || c >= E - LO; }
 };
 
 template <int N, int MODE, int MUL, int FL, int RV>
-__global__ void __launch_bounds__((RowCfg<N, RV>::LINES * Plan<N>::T)) k_row(RowArgs a) {
+__global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_row(RowArgs a) {
   using Cfg = RowCfg<N, RV>;
+  using P = typename Cfg::P;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
   using LL = LdsLine<Cfg::SH>;
-  constexpr int T = Plan<N>::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
+  constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
   __shared__ float2 lds[LINES * LS];
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   const long long line = (long long)blockIdx.x * LINES + ln;
@@ -124,10 +137,18 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * Plan<N>::T)) k_row(Row
     }
   }
   const LL L{lds + ln * LS, 1};
-  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<Plan<N>, N, +1, S>(v, t, a.tw, L);
+  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, a.tw, L);
+  if constexpr ((FL & STORE_AUX) != 0) {
+    if (ok) {
+      float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
+#pragma unroll
+      for (int c = 0; c < E; ++c) row[t + c * T] = v[c];
+    }
+  }
   if constexpr (MUL == RM_T) {
     if (ok) {
-      const float2* tr = a.t + (long long)(b / a.t_group) * a.t_bstride + (long long)y * N;
+      const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
+      const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
 #pragma unroll
       for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
     }
@@ -147,7 +168,7 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * Plan<N>::T)) k_row(Row
       for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
     }
   }
-  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<Plan<N>, N, -1, S>(v, t, a.tw, L);
+  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, a.tw, L);
   if (ok) {
     float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
@@ -184,11 +205,11 @@ template <int MODE> struct ColModeTraits {
 };
 
 // FFT / multiply / IFFT of one column held canonically by T threads
-template <int N, int MODE, int MUL, class S, class LL>
-__device__ __forceinline__ void col_body(float2 (&v)[N / Plan<N>::T], int t, int x, const ColArgs& a, const LL& L) {
+template <class P, int N, int MODE, int MUL, class S, class LL>
+__device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, const ColArgs& a, const LL& L) {
   using Tr = ColModeTraits<MODE>;
-  constexpr int T = Plan<N>::T, E = N / T;
-  if constexpr (Tr::FWD) fft_line<Plan<N>, N, -1, S>(v, t, a.tw, L);
+  constexpr int T = P::T, E = N / T;
+  if constexpr (Tr::FWD) fft_line<P, N, -1, S>(v, t, a.tw, L);
   // frequencies are read only by modes that need them (scale-only passes may
   // run without k-grids)
   constexpr bool NEEDK = (Tr::MULT && MUL != CM_SCALE);
@@ -206,7 +227,7 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / Plan<N>::T], int t, int
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
   }
-  if constexpr (Tr::INV) fft_line<Plan<N>, N, +1, S>(v, t, a.tw, L);
+  if constexpr (Tr::INV) fft_line<P, N, +1, S>(v, t, a.tw, L);
 }
 
 // variants 0, 1: the C columns of a tile spread over the workgroup, thread
@@ -214,7 +235,7 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / Plan<N>::T], int t, int
 template <int N, int MODE, int MUL, int CV>
 __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
   using Cfg = ColCfg<N, CV>;
-  constexpr int T = Plan<N>::T, E = N / T, C = Cfg::C;
+  constexpr int T = Cfg::T, E = N / T, C = Cfg::C;
   __shared__ float2 lds[Cfg::LDS];
   const int col = threadIdx.x % C, t = threadIdx.x / C;
   // band-limited runs launch only the tiles at both ends of kx
@@ -230,8 +251,8 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
     for (int c = 0; c < E; ++c) v[c] = src[(long long)(t + c * T) * nx];
   }
   using LL = LdsLine<Cfg::SH>;
-  const LL L = (CV == 0) ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
-  col_body<N, MODE, MUL, BlockSync>(v, t, x, a, L);
+  const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
+  col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, L);
   if constexpr (MODE == COL_FWD_DETECT) {
     const float kxv = a.kx[x];
     float acc = 0.f;
@@ -266,11 +287,12 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
 // ------------------------------------------------------------------ dispatch
 template <int N, int RV>
 static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
-  constexpr int LINES = RowCfg<N, RV>::LINES, T = Plan<N>::T;
+  constexpr int LINES = RowCfg<N, RV>::LINES, T = RowCfg<N, RV>::T;
   const long long nlines = (long long)a.nbatch * a.ny;
   if (nlines <= 0) return hipSuccess;
   dim3 grid((unsigned)((nlines + LINES - 1) / LINES)), block(LINES * T);
-  const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0);
+  const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0) | (a.aux ? STORE_AUX : 0);
+  if (a.aux && mode != ROW_INV_MUL_FWD) return hipErrorInvalidValue;
 #define MSL_ROW(M, U, F) hipLaunchKernelGGL((k_row<N, M, U, F, RV>), grid, block, 0, s, a)
   if (mode == ROW_FWD && mul == RM_NONE && fl == 0) MSL_ROW(ROW_FWD, RM_NONE, 0);
   else if (mode == ROW_FWD && mul == RM_NONE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_NONE, PRUNE_OUT);
@@ -287,6 +309,9 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == 0) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, 0);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == (PRUNE_IN | PRUNE_OUT))
     MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, PRUNE_IN | PRUNE_OUT);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == STORE_AUX) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, STORE_AUX);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == (PRUNE_IN | PRUNE_OUT | STORE_AUX))
+    MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, PRUNE_IN | PRUNE_OUT | STORE_AUX);
   else return hipErrorInvalidValue;
 #undef MSL_ROW
   return hipGetLastError();
@@ -294,6 +319,7 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
 
 template <int N>
 static hipError_t row_n(int mode, int mul, const RowArgs& a, hipStream_t s, int var) {
+  if (var == 2) return row_nv<N, 2>(mode, mul, a, s);
   if (var == 1) return row_nv<N, 1>(mode, mul, a, s);
   return row_nv<N, 0>(mode, mul, a, s);
 }
@@ -304,10 +330,16 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
   constexpr int C = Cfg::C;
   if (a.nbatch <= 0) return hipSuccess;
   if (a.nx % C != 0) return hipErrorInvalidValue;
-  if (a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / C) return hipErrorInvalidValue;
-  dim3 grid(a.tiles_lo > 0 ? 2 * a.tiles_lo : a.nx / C, a.nbatch), block(Cfg::THREADS);
+  if (a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / ColTile0<N>::C) return hipErrorInvalidValue;
+  // tiles_lo counts tiles of the default width; convert to this variant's
+  ColArgs b = a;
+  if (a.tiles_lo > 0) {
+    if ((a.nx / 4) % C != 0) return hipErrorInvalidValue;
+    b.tiles_lo = a.nx / 4 / C;
+  }
+  dim3 grid(b.tiles_lo > 0 ? 2 * b.tiles_lo : a.nx / C, a.nbatch), block(Cfg::THREADS);
   {
-#define MSL_COL(M, U) hipLaunchKernelGGL((k_col<N, M, U, CV>), grid, block, 0, s, a)
+#define MSL_COL(M, U) hipLaunchKernelGGL((k_col<N, M, U, CV>), grid, block, 0, s, b)
     if (mode == COL_FWD) MSL_COL(COL_FWD, CM_SCALE);
     else if (mode == COL_INV) MSL_COL(COL_INV, CM_SCALE);
     else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COL(COL_FWD_MUL_INV, CM_PROP);
@@ -328,8 +360,10 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
 
 template <int N>
 static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
-  // the detector pass reduces over the workgroup in the interleaved layout
-  if (mode == COL_FWD_DETECT) return col_nv<N, 0>(mode, mul, a, s);
+  // the detector pass reduces over the workgroup in an interleaved layout
+  if (mode == COL_FWD_DETECT) return var == 2 ? col_nv<N, 2>(mode, mul, a, s) : col_nv<N, 0>(mode, mul, a, s);
+  if (var == 3) return col_nv<N, 3>(mode, mul, a, s);
+  if (var == 2) return col_nv<N, 2>(mode, mul, a, s);
   if (var == 1) return col_nv<N, 1>(mode, mul, a, s);
   return col_nv<N, 0>(mode, mul, a, s);
 }

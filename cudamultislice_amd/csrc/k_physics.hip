@@ -605,7 +605,7 @@ hipError_t launch_sum_maps(const float* in, int nb, long long n, float* out, int
 // ============================================================== FD
 // finiteDifferenceKernel, U.cu:995-1023 (pi written 3.14159f there)
 __global__ void k_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi, const float2* V,
-                            long long n, float sigma, float wavel, float dz) {
+                            long long n, long long vper, float sigma, float wavel, float dz) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float a = 2.f * 3.14159f * dz / wavel;
@@ -613,13 +613,15 @@ __global__ void k_fd_update(const float2* grad, const float2* psim, float2* psip
   const float2 rcp = make_float2(1.f / den, -a / den);                     // 1 / (1 + i a)
   const float2 ratio = make_float2((1.f - a * a) / den, -2.f * a / den);   // (1 - i a) / (1 + i a)
   const float2 p = psi[i], g = grad[i], m = psim[i];
-  const float vv = dz * dz * 4.f * 3.14159f * sigma * V[i].x / wavel;
+  const float vv = dz * dz * 4.f * 3.14159f * sigma * V[i % vper].x / wavel;
   const float2 A = make_float2(2.f * p.x - dz * dz * g.x - vv * p.x, 2.f * p.y - dz * dz * g.y - vv * p.y);
   psip[i] = csub(cmul(rcp, A), cmul(ratio, m));
 }
 hipError_t launch_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi, const float2* V,
-                            long long n, float sigma, float wavel, float dz, hipStream_t s) {
-  hipLaunchKernelGGL(k_fd_update, dim3(nblk(n, 256)), dim3(256), 0, s, grad, psim, psip, psi, V, n, sigma, wavel, dz);
+                            long long n, long long vper, float sigma, float wavel, float dz, hipStream_t s) {
+  if (vper <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_fd_update, dim3(nblk(n, 256)), dim3(256), 0, s, grad, psim, psip, psi, V, n, vper, sigma, wavel,
+                     dz);
   return hipGetLastError();
 }
 

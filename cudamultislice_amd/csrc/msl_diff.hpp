@@ -6,7 +6,7 @@ namespace msl {
 
 struct AdjArgs {
   const float2* G;   // gradient w.r.t. the row-pass output a_i, (kx, y) domain
-  const float2* Z;   // z_{i-1} in the (kx, y) domain, or psi0 in real space for the first slice
+  const float2* Z;   // real_z: w_i = psi_{i-1} in real space; else z_{i-1} in the (kx, y) domain
   float2* Gout;      // gradient w.r.t. z_{i-1} (may alias G)
   const float* V;    // potential of slice i, [ny][nx]
   float sigma;
@@ -20,6 +20,6 @@ struct AdjArgs {
 };
 
 int adj_lines(int nx);
-hipError_t launch_row_adjoint(const AdjArgs& a, bool first, hipStream_t s);
+hipError_t launch_row_adjoint(const AdjArgs& a, bool real_z, hipStream_t s);
 
 }  // namespace msl

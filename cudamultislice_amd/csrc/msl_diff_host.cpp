@@ -5,6 +5,8 @@
 // Backward: adjoint passes in reverse slice order; every K-th intermediate
 // wave is checkpointed by the forward and the waves inside a segment are
 // recomputed from its checkpoint (the last segment is kept from the forward).
+// The passes that produce a segment store its real-space waves psi_{i-1}, so
+// the adjoint row pass transforms only the gradient.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -63,12 +65,10 @@ struct msl_diff {
 
 namespace {
 
-// location of z_j (j >= 1) during the forward / inside a recomputed segment
-float2* zloc(msl_diff* d, int j, int seg_base) {
-  if (j % d->K == 0 && j < d->ns) return d->ckpt.p + (size_t)(j / d->K - 1) * d->B * d->np();
-  if (j > seg_base && j < seg_base + d->K) return d->seg.p + (size_t)(j - seg_base - 1) * d->B * d->np();
-  return d->W.p;
-}
+// checkpoint z_j (j a positive multiple of K below ns)
+float2* ckpt_slot(msl_diff* d, int j) { return d->ckpt.p + (size_t)(j / d->K - 1) * d->B * d->np(); }
+// real-space wave w_i = psi_{i-1} of slice i in the segment starting at base
+float2* wslot(msl_diff* d, int i, int base) { return d->seg.p + (size_t)(i - base - 1) * d->B * d->np(); }
 
 RowArgs rargs(msl_diff* d, const float2* src, float2* dst, bool pin = false, bool pout = false) {
   RowArgs a{};
@@ -88,10 +88,13 @@ ColArgs cargs(msl_diff* d, const float2* src, float2* dst, float scale) {
   return a;
 }
 
-// slice i (1-based) forward: z_{i-1} (or psi0) -> z_i at dst
-void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, hipStream_t s) {
+// slice i (1-based) forward: z_{i-1} (psi0 in real space for i = 1) -> z_i
+// at dst, through W (dst may be W: both passes work in place).  With wout
+// the row pass also stores w_i = IF_x(z_{i-1}) there.
+void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, float2* wout, hipStream_t s) {
   RowArgs r = rargs(d, zprev, d->W.p, i > 1, true);
   r.V = d->V + (size_t)(i - 1) * d->np();
+  if (i > 1) r.aux = wout;
   HC(launch_row(i == 1 ? ROW_FWD : ROW_INV_MUL_FWD, RM_PHASE, r, s));
   HC(launch_col(COL_FWD_MUL_INV, CM_PROP, cargs(d, d->W.p, dst, d->nf * d->nf), s));
 }
@@ -158,7 +161,7 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     d->G.alloc(wave);
     d->exitw.alloc(wave);
     if (d->S > 1) d->ckpt.alloc((size_t)(d->S - 1) * wave);
-    if (d->K > 1) d->seg.alloc((size_t)(d->K - 1) * wave);
+    d->seg.alloc((size_t)d->K * wave);
     // columns outside the band are never written by pruned passes: keep them
     // finite from the start
     for (Buf<float2>* b : {&d->W, &d->G, &d->exitw, &d->ckpt, &d->seg})
@@ -195,7 +198,7 @@ msl_status msl_diff_info(const msl_diff* d, float* wl, float* sg, float* km, lon
   if (km) *km = d->kmax;
   if (ws) {
     const long long w = (long long)d->B * d->np() * 8;
-    *ws = w * (3 + (d->S - 1) + (d->K - 1)) + (long long)d->ngroups * d->np() * 4;
+    *ws = w * (3 + (d->S - 1) + d->K) + (long long)d->ngroups * d->np() * 4;
   }
   return MSL_OK;
 }
@@ -207,11 +210,13 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
     hipStream_t s = stream ? (hipStream_t)stream : d->own;
     d->V = V;
     d->psi0 = (const float2*)psi0;
-    const int last_base = (d->S - 1) * d->K;  // the last segment's waves stay in seg
+    // z_K, z_2K, ... go to the checkpoints; the last segment's real-space
+    // waves w_i stay in seg for the backward pass
+    const int last_base = (d->S - 1) * d->K;
     const float2* zprev = d->psi0;
     for (int i = 1; i <= d->ns; ++i) {
-      float2* dst = zloc(d, i, last_base);
-      fwd_slice(d, i, zprev, dst, s);
+      float2* dst = (i % d->K == 0 && i < d->ns) ? ckpt_slot(d, i) : d->W.p;
+      fwd_slice(d, i, zprev, dst, i > last_base ? wslot(d, i, last_base) : nullptr, s);
       zprev = dst;
     }
     HC(launch_row(ROW_INV, RM_NONE, rargs(d, zprev, d->exitw.p, true, false), s));
@@ -241,13 +246,14 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
     for (int sg = d->S - 1; sg >= 0; --sg) {
       const int base = sg * d->K;
       const int top = std::min(base + d->K, d->ns);
-      if (sg < d->S - 1) {
-        // recompute z_{base+1} .. z_{top-1} from the checkpoint
-        const float2* zprev = base == 0 ? d->psi0 : zloc(d, base, base);
+      const bool kept = sg == d->S - 1;  // w_i of the last segment come from the forward
+      // z_{top-1} for the adjoint of slice top in recomputed segments
+      const float2* ztop = base == 0 ? d->psi0 : ckpt_slot(d, base);
+      if (!kept) {
+        // recompute z_{base+1} .. z_{top-1} from the checkpoint, storing w_i
         for (int j = base + 1; j < top; ++j) {
-          float2* dst = zloc(d, j, base);
-          fwd_slice(d, j, zprev, dst, s);
-          zprev = dst;
+          fwd_slice(d, j, ztop, d->W.p, wslot(d, j, base), s);
+          ztop = d->W.p;
         }
       }
       for (int i = top; i > base; --i) {
@@ -255,7 +261,13 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
         AdjArgs a{};
         a.G = d->G.p;
         a.Gout = d->G.p;
-        a.Z = (i == 1) ? d->psi0 : zloc(d, i - 1, base);
+        bool real_z = true;
+        if (i == 1) a.Z = d->psi0;
+        else if (kept || i < top) a.Z = wslot(d, i, base);
+        else {
+          a.Z = ztop;  // z_{top-1}: w_top is transformed inside the adjoint pass
+          real_z = false;
+        }
         a.V = d->V + (size_t)(i - 1) * np;
         a.sigma = d->sigma;
         a.bstride = np;
@@ -265,7 +277,7 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
         a.partial = d->partial.p;
         a.tw = d->twx.p;
         a.prune = d->prune_tiles > 0;
-        HC(launch_row_adjoint(a, i == 1, s));
+        HC(launch_row_adjoint(a, real_z, s));
         HC(launch_sum_maps(d->partial.p, d->ngroups, np, gradV + (size_t)(i - 1) * np, 0, s));
       }
     }

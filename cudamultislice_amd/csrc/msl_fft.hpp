@@ -111,6 +111,19 @@ template <> struct Plan<512> { static constexpr int T = 64, R1 = 8, R2 = 8, R3 =
 template <> struct Plan<1024> { static constexpr int T = 64, R1 = 16, R2 = 8, R3 = 8; };
 template <> struct Plan<2048> { static constexpr int T = 128, R1 = 16, R2 = 16, R3 = 8; };
 
+// Alternative plans: a different thread count per line and radix order (more
+// butterflies per thread and stage, or fewer threads), selectable per length
+// as pass variants.
+template <int N> struct AltPlan;
+template <> struct AltPlan<16> { static constexpr int T = 2, R1 = 8, R2 = 2, R3 = 1; };
+template <> struct AltPlan<32> { static constexpr int T = 8, R1 = 4, R2 = 4, R3 = 2; };
+template <> struct AltPlan<64> { static constexpr int T = 4, R1 = 16, R2 = 4, R3 = 1; };
+template <> struct AltPlan<128> { static constexpr int T = 16, R1 = 8, R2 = 8, R3 = 2; };
+template <> struct AltPlan<256> { static constexpr int T = 32, R1 = 8, R2 = 8, R3 = 4; };
+template <> struct AltPlan<512> { static constexpr int T = 32, R1 = 16, R2 = 8, R3 = 4; };
+template <> struct AltPlan<1024> { static constexpr int T = 32, R1 = 16, R2 = 8, R3 = 8; };
+template <> struct AltPlan<2048> { static constexpr int T = 64, R1 = 16, R2 = 16, R3 = 8; };
+
 // Exchange synchronisation: a whole workgroup, or one wavefront when every
 // line lives inside a single wave (LDS traffic of one wave is processed in
 // order; the fences keep the compiler from moving accesses across).

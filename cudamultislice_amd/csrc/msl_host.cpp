@@ -20,47 +20,12 @@
 #include "fparams_table.h"
 #include "msl_internal.hpp"
 
+#include "msl_host.hpp"
+
 using namespace msl;
+using namespace mslh;
 
-// ------------------------------------------------------------------ errors
-namespace {
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-inline void hip_check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return;
-  if (e == hipErrorOutOfMemory) throw Error(MSL_ERR_OOM, std::string(what) + ": out of device memory");
-  throw Error(MSL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPC(x) hip_check((x), #x)
-
-inline void require(bool ok, int code, const std::string& msg) {
-  if (!ok) throw Error(code, msg);
-}
-
-template <class T> struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void ensure(size_t count) {
-    if (count <= n && p) return;
-    release();
-    if (count == 0) return;
-    HIPC(hipMalloc(&p, count * sizeof(T)));
-    n = count;
-  }
-};
+namespace mslh {
 
 bool pow2_ok(int n) { return fft_size_supported(n); }
 
@@ -132,75 +97,9 @@ std::vector<float2> twiddles(int n) {
   return t;
 }
 
-}  // namespace
+}  // namespace mslh
 
-// ------------------------------------------------------------------ handle
-struct msl_sim {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  unsigned compat = MSL_COMPAT_REFERENCE_SYMBOLS;
-  int potmode = MSL_POT_INTEGRATED;
-  volatile int cancel = 0;
-
-  // structure (U.cuh:88-99, 127-132)
-  std::vector<int> Z;
-  std::vector<float> x, y, z;
-  float MaxX = 0, MinX = 0, MaxY = 0, MinY = 0, MaxZ = 0, MinZ = 0;
-  bool have_atoms = false;
-
-  // parameterisation
-  std::vector<float> fparams = std::vector<float>(kFparams, kFparams + 103 * 12);
-  bool have_params = false;
-  DBuf<float> ztab;
-
-  // binned atoms on the device
-  DBuf<float> in_x, in_y, in_z, bx, by, bz;
-  DBuf<int> in_Z, bZ, key, count, start, fill, perm, bad, scan_tmp;
-  int nslab = 0, nbinned = 0;
-  bool binned = false;
-  uint64_t tds_seed = 0x4D534CULL;
-  float tds_sigma = 0.075f;  // TDSRand, U.cu:34
-  int tds_config = 0;
-
-  // microscope (U.cuh:101-111)
-  float kV = 0, df = 0, A1 = 0, A1phi = 0, A2 = 0, A2phi = 0, Cs = 0, beta = 0, delta = 0, obj = 0;
-  bool have_micro = false;
-
-  // calculation variables (U.cuh:113-125)
-  int resX = 0, resY = 0;
-  float ps = 0, offx = 0, offy = 0, ksizex = 0, ksizey = 0;
-  float wavel = 0, sigma2 = 0, dzFD = 0;
-  int numberOfSlices = 0, loadblocksx = 0, loadblocksy = 0, loadslicesz = 0;
-  bool have_calc = false, have_freqs = false;
-  float kmax = 0, nf = 0;
-  int prune_tiles = 0;  // > 0: columns N/4 <= x < 3N/4 are outside the propagator band
-  std::vector<float> hkx, hky;
-  DBuf<float> kx, ky;
-  DBuf<float2> twx, twy, px, py;
-  float prop_dz = std::nanf("");
-  float conventionaldz = 1.0f;
-
-  // waves
-  DBuf<float2> PsiMinus, Psi, PsiPlus, V, G, B;
-  bool have_waves = false, fd_ready = false;
-  DBuf<float> tds, image, red;
-  bool have_tds = false, have_image = false;
-  float maxposIm = 0, maxposEW = 0, minposEW = 0, minposIm = 0, imagecontrast = 0;
-  float timings[4] = {0, 0, 0, 0};
-  // per-kernel HIP-event timing of batched runs: [row ms, row n, col ms,
-  // col n, detect ms, detect n, potential ms, potential n]
-  bool profiling = false;
-  double kstats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  std::vector<hipEvent_t> events;
-
-  size_t npix() const { return (size_t)resX * resY; }
-  ~msl_sim() {
-    for (auto e : events) (void)hipEventDestroy(e);
-  }
-};
-
-namespace {
+namespace mslh {
 
 void set_device(msl_sim* s) { HIPC(hipSetDevice(s->device)); }
 
@@ -550,29 +449,8 @@ float adf_of(msl_sim* s, const float2* w, float inner, float outer) {
   return download_scalar(s, s->red.p + 2048);
 }
 
-}  // namespace
+}  // namespace mslh
 
-// ------------------------------------------------------------------ C-ABI
-#define API_BEGIN(h)                                        \
-  if (!(h)) return MSL_ERR_ARG;                             \
-  try {                                                     \
-    set_device(h);
-#define API_END(h)                                          \
-  }                                                         \
-  catch (const Error& e) {                                  \
-    (h)->err = e.what();                                    \
-    return e.code;                                          \
-  }                                                         \
-  catch (const std::bad_alloc&) {                           \
-    (h)->err = "host allocation failed";                    \
-    return MSL_ERR_OOM;                                     \
-  }                                                         \
-  catch (const std::exception& e) {                         \
-    (h)->err = e.what();                                    \
-    return MSL_ERR_ARG;                                     \
-  }                                                         \
-  (h)->err.clear();                                         \
-  return MSL_OK;
 
 extern "C" {
 
@@ -613,7 +491,7 @@ void msl_destroy(msl_sim* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) {
     (void)hipStreamSynchronize(s->stream);
-    (void)hipStreamDestroy(s->stream);
+    if (s->own_stream) (void)hipStreamDestroy(s->stream);
   }
   delete s;
 }
@@ -883,7 +761,7 @@ msl_status msl_multislice_step(msl_sim* s, int it) {
   c.scale = s->nf * s->nf;
   col(s, COL_FWD_MUL_INV, CM_GRAD, c);
   row(s, ROW_INV, RM_NONE, row_args(s, s->G.p, s->G.p, 1, bs));
-  HIPC(launch_fd_update(s->G.p, s->PsiMinus.p, s->PsiPlus.p, s->Psi.p, s->V.p, bs, s->sigma2, s->wavel, s->dzFD,
+  HIPC(launch_fd_update(s->G.p, s->PsiMinus.p, s->PsiPlus.p, s->Psi.p, s->V.p, bs, bs, s->sigma2, s->wavel, s->dzFD,
                         s->stream));
   // band-limit psi+
   row(s, ROW_FWD, RM_NONE, row_args(s, s->PsiPlus.p, s->B.p, 1, bs));
@@ -1218,6 +1096,73 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
   const int l1chunks = (int)((np + 4095) / 4096);
   const float norm = download_scalar(s, s->red.p + l1chunks);
   require(norm > 0 && std::isfinite(norm), MSL_ERR_RANGE, "probe has zero amplitude (aperture too small?)");
+  const bool want_exit = d->exit_out != nullptr;
+  const bool prune = s->prune_tiles > 0;
+  // detector tiles per probe: the |Psi|^2 map needs every column written
+  const int det_tiles = (prune && !d->pacbed_out) ? 2 * s->prune_tiles : ntiles;
+  // One chunk of nb probes whose positions are in pos: probe synthesis, the
+  // slice loop and the fused detector.  Every pointer it uses is fixed for
+  // the whole call, so the sequence can be captured once per chunk size and
+  // replayed (no memsets inside).
+  auto enqueue_chunk = [&](int nb, KTimer* kt) {
+    // probes in k-space, then IFFT_y with the 1/sum|psi| scale -> (kx, y) domain
+    ProbeArgs p{};
+    p.out = W.p; p.bstride = np; p.nx = nx; p.ny = ny; p.nprobe = nb; p.kx = s->kx.p; p.ky = s->ky.p;
+    p.posx = pos.p; p.posy = pos.p + B;
+    p.apert = (float)((double)s->obj * 0.001 / (double)s->wavel);
+    p.ps = s->ps; p.df = s->df; p.Cs = s->Cs; p.wavel = s->wavel;
+    HIPC(launch_probe_kspace(p, s->stream));
+    ColArgs ci = col_args(s, W.p, W.p, nb, np);
+    ci.scale = 1.0f / norm;
+    col(s, COL_INV, CM_SCALE, ci);
+    for (int i = 1; i <= nslices; ++i) {
+      RowArgs r = row_args(s, W.p, W.p, nb, np);
+      r.t = tall.p + (size_t)(i - 1) * np;
+      r.t_group = nb;
+      r.t_bstride = 0;
+      r.prune_in = prune && i > 1;  // the probe itself is not band-limited
+      r.prune_out = prune;
+      if (kt) kt->begin(0);
+      row(s, ROW_INV_MUL_FWD, RM_T, r);
+      if (kt) kt->end();
+      ColArgs c = col_args(s, W.p, W.p, nb, np);
+      c.tiles_lo = s->prune_tiles;
+      if (i < nslices || want_exit) {
+        c.scale = s->nf * s->nf;
+        if (kt) kt->begin(1);
+        col(s, COL_FWD_MUL_INV, CM_PROP, c);
+        if (kt) kt->end();
+      } else {
+        c.dst = nullptr;
+        c.scale = 1.0f;
+        c.kin = d->adf_inner * 1e-3f / s->wavel;
+        c.kout = d->adf_outer * 1e-3f / s->wavel;
+        c.det_scale = s->nf * s->nf;
+        c.partial = partial.p;
+        c.inten = d->pacbed_out ? inten.p : nullptr;
+        if (d->pacbed_out) c.tiles_lo = 0;
+        if (kt) kt->begin(2);
+        col(s, COL_FWD_DETECT, CM_PROP, c);
+        if (kt) kt->end();
+      }
+    }
+  };
+  // graphs replace the eager launches unless exit waves are copied out per
+  // chunk or per-kernel events are being recorded
+  const bool graph_mode = d->use_graph != 0 && !want_exit && !s->profiling;
+  struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    int nb = -1;
+    void reset() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+      e = nullptr;
+      g = nullptr;
+      nb = -1;
+    }
+    ~Graph() { reset(); }
+  } graph;
   double tb = 0, tp = 0, tprop = 0;
   std::vector<float> dx, dy, dzv;
   for (int cfg = 0; cfg < nconf; ++cfg) {
@@ -1247,53 +1192,27 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
       const int nb = std::min(B, d->nprobe - p0);
       HIPC(hipMemcpyAsync(pos.p, d->posx + p0, nb * sizeof(int), hipMemcpyHostToDevice, s->stream));
       HIPC(hipMemcpyAsync(pos.p + B, d->posy + p0, nb * sizeof(int), hipMemcpyHostToDevice, s->stream));
-      // probes in k-space, then IFFT_y with the 1/sum|psi| scale -> (kx, y) domain
-      ProbeArgs p{};
-      p.out = W.p; p.bstride = np; p.nx = nx; p.ny = ny; p.nprobe = nb; p.kx = s->kx.p; p.ky = s->ky.p;
-      p.posx = pos.p; p.posy = pos.p + B;
-      p.apert = (float)((double)s->obj * 0.001 / (double)s->wavel);
-      p.ps = s->ps; p.df = s->df; p.Cs = s->Cs; p.wavel = s->wavel;
-      HIPC(launch_probe_kspace(p, s->stream));
-      ColArgs ci = col_args(s, W.p, W.p, nb, np);
-      ci.scale = 1.0f / norm;
-      col(s, COL_INV, CM_SCALE, ci);
-      const bool want_exit = d->exit_out != nullptr;
-      const bool prune = s->prune_tiles > 0;
-      int det_tiles = ntiles;
-      for (int i = 1; i <= nslices; ++i) {
-        RowArgs r = row_args(s, W.p, W.p, nb, np);
-        r.t = tall.p + (size_t)(i - 1) * np;
-        r.t_group = nb;
-        r.t_bstride = 0;
-        r.prune_in = prune && i > 1;  // the probe itself is not band-limited
-        r.prune_out = prune;
-        kt.begin(0);
-        row(s, ROW_INV_MUL_FWD, RM_T, r);
-        kt.end();
-        ColArgs c = col_args(s, W.p, W.p, nb, np);
-        c.tiles_lo = s->prune_tiles;
-        if (i < nslices || want_exit) {
-          c.scale = s->nf * s->nf;
-          kt.begin(1);
-          col(s, COL_FWD_MUL_INV, CM_PROP, c);
-          kt.end();
-        } else {
-          c.dst = nullptr;
-          c.scale = 1.0f;
-          c.kin = d->adf_inner * 1e-3f / s->wavel;
-          c.kout = d->adf_outer * 1e-3f / s->wavel;
-          c.det_scale = s->nf * s->nf;
-          c.partial = partial.p;
-          c.inten = d->pacbed_out ? inten.p : nullptr;
-          // the |Psi|^2 map needs every column written
-          if (d->pacbed_out) c.tiles_lo = 0;
-          det_tiles = c.tiles_lo > 0 ? 2 * c.tiles_lo : ntiles;
-          kt.begin(2);
-          col(s, COL_FWD_DETECT, CM_PROP, c);
-          kt.end();
+      if (graph_mode) {
+        if (graph.nb != nb) {
+          graph.reset();
+          HIPC(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+          try {
+            enqueue_chunk(nb, nullptr);
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          HIPC(hipStreamEndCapture(s->stream, &graph.g));
+          HIPC(hipGraphInstantiate(&graph.e, graph.g, nullptr, nullptr, 0));
+          graph.nb = nb;
         }
+        HIPC(hipGraphLaunch(graph.e, s->stream));
+      } else {
+        enqueue_chunk(nb, &kt);
+        kt.flush();
       }
-      kt.flush();
       if (want_exit) {
         RowArgs ri = row_args(s, W.p, W.p, nb, np);
         ri.prune_in = prune;

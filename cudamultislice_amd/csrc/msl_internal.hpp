@@ -20,7 +20,8 @@ struct RowArgs {
   float2* dst;
   long long src_bstride, dst_bstride;  // elements between consecutive waves
   int nx, ny, nbatch;
-  const float2* t;        // RM_T: transmission of wave b at t + (b / t_group) * t_bstride
+  const float2* t;        // RM_T: transmission of wave b at t + (b / t_group) * t_bstride,
+                          // or at t + t_index[b] * t_bstride when t_index is set
   const float* V;         // RM_PHASE: potential, same indexing
   long long t_bstride;
   int t_group;
@@ -28,6 +29,9 @@ struct RowArgs {
   float scale;            // applied with the pointwise factor
   int prune_in, prune_out; // band-limited input / output (only x < N/4, x >= 3N/4)
   const float2* tw;       // twiddles for length nx
+  const int* t_index;     // optional per-wave transmission slot (device)
+  float2* aux;            // ROW_INV_MUL_FWD: also store the real-space wave (before the
+                          // multiply) here, same indexing as dst (null: not stored)
 };
 
 // Column pass: one transform along y (stride nx) per (wave, column).
@@ -164,8 +168,11 @@ hipError_t launch_tds_accum(const float2* F, float* tds, int nx, int ny, hipStre
 hipError_t launch_sum_maps(const float* in, int nbatch, long long n, float* out, int accumulate, hipStream_t s);
 
 // Finite-difference update (finiteDifferenceKernel, U.cu:995-1023)
+// psi+ of n points; V is periodic with period vper (one potential for a
+// batch of waves; vper = n: one per wave)
 hipError_t launch_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi,
-                            const float2* V, long long n, float sigma, float wavel, float dz, hipStream_t s);
+                            const float2* V, long long n, long long vper, float sigma, float wavel, float dz,
+                            hipStream_t s);
 
 // differentiable multislice helpers
 hipError_t launch_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n, int nbatch,

@@ -54,11 +54,14 @@ struct Bench {
     return hipStreamSynchronize(s);
   }
   hipError_t fill(float2* w, int nw, hipStream_t s) { return launch_hash_fill(w, (long long)nw * n * n, 11u, 0, s); }
-  // kind 0: row IFFT * t * FFT (scaled 1/n); kind 1: column FFT * P / n^2 * IFFT
+  // kind 0: row IFFT * t * FFT (scaled 1/n); kind 1: column FFT * P / n^2 * IFFT;
+  // band-pruned as in a multislice run wherever the length allows it
   hipError_t run(int kind, int var, float2* w, int nw, hipStream_t s) {
     const long long bs = (long long)n * n;
+    const int tiles_lo = band_tiles_lo(n);
     if (kind == 0) {
       RowArgs a{};
+      a.prune_in = a.prune_out = tiles_lo > 0;
       a.src = w; a.dst = w; a.src_bstride = a.dst_bstride = bs; a.nx = n; a.ny = n; a.nbatch = nw;
       a.t = t.as<float2>(); a.t_group = nw; a.t_bstride = 0; a.scale = 1.0f / n; a.tw = tw.as<float2>();
       return launch_row_v(ROW_INV_MUL_FWD, RM_T, a, s, var);
@@ -67,6 +70,7 @@ struct Bench {
     c.src = w; c.dst = w; c.src_bstride = c.dst_bstride = bs; c.nx = n; c.ny = n; c.nbatch = nw;
     c.kx = k.as<float>(); c.ky = k.as<float>(); c.px = pxy.as<float2>(); c.py = pxy.as<float2>();
     c.kmax = kmax; c.scale = 1.0f / ((float)n * n); c.tw = tw.as<float2>();
+    c.tiles_lo = tiles_lo;
     return launch_col_v(COL_FWD_MUL_INV, CM_PROP, c, s, var);
   }
 };
@@ -153,11 +157,13 @@ msl_status msl_bench_pass(int kind, int variant, int n, int nbatch, int reps, fl
 
 // Autotuner: for length n and a batch of nbatch waves, time every row and
 // column variant that agrees with variant 0 to 1e-5 and select the fastest.
-// out (optional): [row ms per variant (3)], [column ms per variant (4)],
-// -1 for variants rejected or not built for this length.
+// out (optional): [0..3] row ms per variant, [4..7] column ms per variant,
+// -1 for variants rejected or not built.
 msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* col_var, float* out) {
   if (!fft_size_supported(n) || nbatch < 1 || reps < 1) return MSL_ERR_ARG;
   int best[2] = {0, 0};
+  if (out)
+    for (int i = 0; i < 8; ++i) out[i] = -1.f;
   for (int kind = 0; kind < 2; ++kind) {
     float bms = 1e30f;
     for (int v = 0; v < num_pass_variants(kind); ++v) {
@@ -165,7 +171,7 @@ msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* c
       msl_status st = msl_bench_pass(kind, v, n, nbatch, reps, &ms, &md);
       if (st != MSL_OK) return st;
       if (!(md < 1e-5f)) ms = -1;
-      if (out) out[(kind == 0 ? 0 : 3) + v] = ms;
+      if (out && v < 4) out[4 * kind + v] = ms;
       if (ms > 0 && ms < bms) {
         bms = ms;
         best[kind] = v;

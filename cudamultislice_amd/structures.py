@@ -70,6 +70,27 @@ def random_atoms(n, box, zrange=(6, 79), dmin=1.4, seed=SEED, max_tries=200000):
     return Z, pts[:, 0].astype(np.float32), pts[:, 1].astype(np.float32), pts[:, 2].astype(np.float32)
 
 
+def jittered_atoms(n, box, zrange=(6, 79), dmin=1.4, seed=SEED):
+    """n atoms spread uniformly over box with every pair at least dmin apart,
+    in O(n): a random subset of the sites of a cubic lattice of spacing a
+    (the largest spacing that still offers n sites), each site moved by an
+    independent uniform offset in [-(a - dmin)/2, (a - dmin)/2] per axis, so
+    atoms on different sites differ by at least dmin along some axis."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    box = np.asarray(box, float)
+    a = (box.prod() / n) ** (1.0 / 3.0)
+    while np.prod(np.maximum(np.floor(box / a), 1)) < n:
+        a *= 0.99
+    if a <= dmin:
+        raise ValueError(f"{n} atoms do not fit in {box.tolist()} at a minimum distance of {dmin}")
+    m = np.maximum(np.floor(box / a), 1).astype(int)
+    sites = rng.choice(int(np.prod(m)), size=n, replace=False)
+    idx = np.stack(np.unravel_index(sites, m), 1).astype(float)
+    pos = (idx + 0.5) * a + rng.uniform(-(a - dmin) / 2, (a - dmin) / 2, size=(n, 3))
+    Z = rng.integers(zrange[0], zrange[1] + 1, size=n).astype(np.int32)
+    return Z, pos[:, 0].astype(np.float32), pos[:, 1].astype(np.float32), pos[:, 2].astype(np.float32)
+
+
 def amorphous_carbon(box=(102.4, 102.4, 200.0), density_per_A3=0.0602, dmin=1.4, seed=SEED):
     n = int(round(density_per_A3 * box[0] * box[1] * box[2]))
     return random_atoms(n, box, (6, 6), dmin, seed)

@@ -154,7 +154,10 @@ typedef struct {
   int nslices;            /* 0: ceil((MaxZ - MinZ) / dz) */
   float adf_inner, adf_outer; /* mrad */
   int batch;              /* probes per launch, 0: automatic */
-  int use_graph;          /* capture the slice loop in a hipGraph */
+  int use_graph;          /* capture one chunk (probe synthesis, slice loop, detector) in a
+                             hipGraph per chunk size and replay it for every chunk and
+                             configuration; ignored with exit_out or while per-kernel
+                             profiling is on */
   float* adf_out;         /* [nprobe], mean over configurations */
   float* pacbed_out;      /* optional [resY][resX]: mean |Psi_hat|^2 (unshifted) */
   float* exit_out;        /* optional [nconfig][nprobe][resY][resX][2] exit waves (small runs) */
@@ -186,11 +189,12 @@ msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, flo
 
 /* ---- pass layouts and autotuning (no counterpart in the reference) ----
  * kind 0: row pass, 1: column pass.  Variants are alternative LDS layouts and
- * synchronisation scopes of the same transform; see DESIGN notes in
+ * synchronisation scopes of the same transform; see the variant notes in
  * k_pass.hip.  msl_bench_pass times `reps` launches on an n x n batch of
  * nbatch waves and, with maxdiff, compares against variant 0;
  * msl_autotune_passes selects the fastest agreeing variant per pass for n
- * (out: 3 row + 4 column timings in ms, -1 where rejected). */
+ * (out[8]: [0..3] row and [4..7] column timings in ms per variant, -1 where
+ * rejected or absent).  Timings use band-pruned passes where the length allows. */
 msl_status msl_set_pass_variant(int kind, int n, int variant);
 msl_status msl_get_pass_variant(int kind, int n, int* variant);
 msl_status msl_bench_pass(int kind, int variant, int n, int nbatch, int reps, float* ms, float* maxdiff);
@@ -198,6 +202,53 @@ msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* c
 /* plain streaming device copy (bytes a multiple of 16); the reference point
  * for hardware traffic counters */
 msl_status msl_copy_device(void* dst, const void* src, long long bytes, void* stream);
+
+/* ---- serving: a mix of independent jobs of uneven size (no counterpart in
+ * the reference, whose GUI runs one simulation at a time, MW.cs:231-403) ----
+ * Each job is the reference's call sequence on its own atoms: extents and
+ * binning (LoadAtomFile / UploadBinnedAtoms), SetCalculationVariables with a
+ * zero block offset (plane wave) or STEMSetCalculationVariables (probes),
+ * PreCalculateFrequencies, then nslices MultisliceStepConv (slice thickness
+ * dz) or nslices finite-difference MultisliceStep, from InitialiseWavefunctions
+ * / STEMInitialiseWavefunctions.  Jobs sharing a grid size (and, for finite
+ * differences, the plane-wave / probe step size) share one batch: every
+ * slice is one row and one column launch over all their waves, the waves of
+ * jobs with fewer slices leave the batch when they finish.  Batches are cut
+ * to a device-memory budget. */
+enum { MSL_METHOD_CONVENTIONAL = 0, MSL_METHOD_FD = 1 };
+typedef struct {
+  int n;                  /* grid n x n, power of two 16..2048 */
+  int method;             /* MSL_METHOD_* */
+  int nslices;            /* conventional slices, or finite-difference steps */
+  int natoms;
+  const int* Z;           /* atomic numbers 1..103 */
+  const float* x;         /* positions (A) */
+  const float* y;
+  const float* z;
+  int nprobe;             /* 0: plane wave; else probes at posx/posy (pixels) */
+  const int* posx;
+  const int* posy;
+  float* exit_out;        /* plane wave: [n][n][2] exit wave (optional) */
+  float* adf_out;         /* probes: [nprobe] ADF, sum |Psi_unitary|^2 in the annulus */
+} msl_job;
+typedef struct {
+  float kV;               /* all jobs */
+  float ps;               /* pixel size (A), all jobs */
+  float dz;               /* conventional slice thickness (A) */
+  float aperture;         /* probe convergence semi-angle (mrad); Cs = df = 0 */
+  float adf_inner, adf_outer; /* mrad */
+  int potential_mode;     /* MSL_POT_* */
+  long long budget_bytes; /* device memory for the waves of one batch (0: 32 GiB) */
+} msl_serve_params;
+typedef struct msl_server msl_server;
+msl_status msl_server_create(msl_server** out, int device);
+void msl_server_destroy(msl_server* srv);
+const char* msl_server_last_error(const msl_server* srv);
+/* runs every job; stats (optional, 8 doubles): total ms, set-up ms (handles,
+ * binning, tables), propagation ms, batches, waves, wave-slices, launches of
+ * the row/column passes, largest batch (bytes) */
+msl_status msl_server_run(msl_server* srv, const msl_serve_params* prm, const msl_job* jobs, int njobs,
+                          double* stats);
 
 /* ---- device-array primitives (used by the torch binding and tests) ---- */
 /* batched 2D FFT of nbatch [ny][nx] complex64 device arrays, in place,

@@ -507,3 +507,17 @@ def test_xyz_reference_symbol_map(tmp_path):
     assert list(s.extents()) == [6.0, -2.0, 7.0, -2.0, 8.25, -2.0]
     p.write_text("2\ncomment line here\nSi 0 0 0\nO 1 1 1\n")
     assert s.load_xyz_refmap(str(p)) == 2
+
+
+def test_jittered_atoms_keep_min_distance_and_fill_box():
+    from scipy.spatial import cKDTree
+    from cudamultislice_amd import structures as S
+    Z, x, y, z = S.jittered_atoms(3000, (30.0, 25.0, 40.0), seed=5)
+    p = np.stack([x, y, z], 1).astype(float)
+    d, _ = cKDTree(p).query(p, k=2)
+    assert d[:, 1].min() >= 1.4 - 1e-5
+    assert p.min(0).min() >= 0 and np.all(p.max(0) <= [30.0, 25.0, 40.0])
+    assert Z.min() >= 6 and Z.max() <= 79 and len(set(Z.tolist())) > 60
+    # spread: every eighth of the box holds atoms
+    occ = np.histogramdd(p, bins=(2, 2, 2), range=((0, 30), (0, 25), (0, 40)))[0]
+    assert occ.min() > 0.08 * len(p)
