@@ -13,8 +13,13 @@ through all slices, ending in the ADF reduction.  The default run times all
 Alongside the headline value the JSON line reports the per-probe launch path
 (the reference API driven one probe at a time, potential rebuilt per slice),
 fwd+bwd/fwd of the differentiable path on 1024^2 x 100 slices x 64 probes,
-the roofline of the dominant kernel and the CPU restatement's throughput on
-a bounded sample.  Diagnostics go to stderr; stdout carries one JSON line.
+two plane-wave TEM cases (Si 256^2 on GPU and CPU; Si 1024^2 x 252 slices
+with an 8-defocus CTF sweep), torch.fft (rocFFT) on the same 512^2 waves as
+a reference point, the roofline of the dominant kernel and the CPU
+restatement's throughput on a bounded sample.  Diagnostics go to stderr;
+stdout carries one JSON line.  --diff-only and --serve JOBS print the line
+of one secondary workload instead (the differentiable path; the uneven
+serving mix).
 """
 import argparse
 import json
@@ -143,6 +148,90 @@ def diff_ratio(reps=3):
             "checkpoint_every": 10, "workspace_GiB": ws / 2**30}
 
 
+def tem_cases(cpu_threads):
+    """Plane-wave TEM through the reference-shaped API: (1) Si 4x4x10 cells,
+    256^2 over 21.72 A, 1 A slices, 200 kV, exit wave + CTF image (df -500 A,
+    Cs 1.2e7 A, 20 mrad), on the GPU and, in full, in the CPU restatement;
+    (2) Si 14x14x92 cells (144 256 atoms), 1024^2 over 76.02 A, 2 A slices,
+    300 kV, exit wave + CTF images for 8 defoci -800..+600 A (Cs 1e4 A,
+    20 mrad) on the GPU."""
+    import numpy as np
+    import torch
+    from cudamultislice_amd import MultisliceSimulation
+    from cudamultislice_amd import structures as S
+    from cudamultislice_amd.fparams import fparams
+    from oracle import oracle as O
+    out = {}
+    for name, cells, N, ps, kV, dz, df, Cs in (("si_4x4x10_256", (4, 4, 10), 256, 0.084844, 200.0, 1.0, -500.0, 1.2e7),
+                                               ("si_14x14x92_1024", (14, 14, 92), 1024, 0.07424, 300.0, 2.0, -800.0, 1e4)):
+        Z, x, y, z = S.silicon(*cells)
+        sim = MultisliceSimulation(0)
+        sim.set_atoms(Z, x, y, z)
+        sim.GetParameterisation()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.UploadBinnedAtoms()
+        sim.ApplyMicroscopeParameters(kV, df, 0.0, 0.0, Cs, 0.0, 0.0, 20.0)
+        inf = sim.info()
+        sim.SetCalculationVariables(ps, df, N, N, inf["MaximumX"] - inf["MinimumX"], inf["MaximumY"] - inf["MinimumY"],
+                                    inf["MaximumZ"] - inf["MinimumZ"], 2.0, 2.0)
+        sim.PreCalculateFrequencies()
+        sim.InitialiseWavefunctions(dz)
+        n = sim.GetSlices(True)
+        for it in range(1, n + 1):
+            sim.MultisliceStepConv(it)
+        ew = sim.GetExitWave()
+        t1 = time.perf_counter()
+        if N == 256:
+            img = sim.SimulateImage(1.0)
+            ndf = 1
+        else:
+            img = sim.ctf_sweep(np.arange(-800, 601, 200, dtype=np.float32))
+            ndf = 8
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        norm = float(np.mean(np.abs(ew) ** 2))
+        assert np.all(np.isfinite(ew)) and np.all(np.isfinite(img)) and 0.0 < norm <= 1.0 + 1e-4
+        rec = {"atoms": int(Z.size), "grid": N, "slices": n, "gpu_ms": (t1 - t0) * 1e3, "ctf_images": ndf,
+               "ctf_ms": (t2 - t1) * 1e3, "exit_mean_intensity": norm}
+        sim.close()
+        if N == 256:
+            o = O.OracleSim(use_float=True)
+            o.set_atoms(Z, x, y, z)
+            o.set_fparams(fparams())
+            O.set_threads(cpu_threads)
+            t0 = time.perf_counter()
+            o.bin()
+            e = o.extents()
+            o.apply_microscope(kV, df, 0.0, 0.0, Cs, 0.0, 0.0, 20.0)
+            o.set_calc(ps, df, N, N, float(e[0] - e[1]), float(e[2] - e[3]), float(e[4] - e[5]), 2.0, 2.0, False)
+            o.init_plane(dz)
+            for it in range(1, n + 1):
+                o.step_conv(it)
+            o.ctf_image(df)
+            rec["cpu_ms"] = (time.perf_counter() - t0) * 1e3
+            rec["cpu_threads"] = cpu_threads
+        out[name] = rec
+        log(f"[tem] {name}: {rec}")
+    return out
+
+
+def rocfft_reference(nbatch=512, N=512, reps=5):
+    """torch.fft.fft2 (rocFFT) forward + inverse on nbatch N^2 complex64
+    waves, as a reference point for the two propagation passes."""
+    import torch
+    x = torch.randn(nbatch, N, N, dtype=torch.complex64, device="cuda")
+    y = torch.fft.ifft2(torch.fft.fft2(x))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y = torch.fft.ifft2(torch.fft.fft2(x))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    del x, y
+    return {"waves": nbatch, "grid": N, "fft2_plus_ifft2_ms": ms, "ms_per_wave": ms / nbatch}
+
+
 def serving_mix(njobs):
     """The uneven serving mix: njobs jobs with grids in {256, 512, 1024,
     2048}^2 (0.1 A pixels), 20-200 slices of 2 A or finite-difference steps,
@@ -206,6 +295,7 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--no-tem", action="store_true", help="skip the plane-wave TEM cases and the rocFFT reference")
     ap.add_argument("--diff-only", action="store_true",
                     help="time only the differentiable path (fwd, fwd+bwd) and print its line")
     ap.add_argument("--serve", type=int, default=0, metavar="JOBS",
@@ -307,6 +397,15 @@ def main():
         diff = diff_ratio()
         log(f"[bench] diff: {diff}")
 
+    tem = rocfft = None
+    if not args.no_tem:
+        tem = tem_cases(min(16, os.cpu_count() or 1))
+        rocfft = rocfft_reference()
+        # the library's two passes for the same work (one slice: the same two
+        # 2D transforms plus the pointwise factors), per wave
+        rocfft["msl_row_plus_column_ms_per_wave"] = (ks["row"]["ms_avg"] + ks["column"]["ms_avg"]) / B
+        log(f"[bench] rocfft reference {rocfft}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -335,6 +434,8 @@ def main():
         "stage_ms_total": tm,
         "per_probe_path": per_probe,
         "fwd_bwd_over_fwd": diff,
+        "tem": tem,
+        "rocfft_reference": rocfft,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -94,6 +94,8 @@ _SIGS = {
     "msl_get_bins": ([_vp, _vp], _i),
     "msl_potential_slice": ([_vp, _i, _f, _i, _vp], _i),
     "msl_set_wave": ([_vp, _i, _vp], _i), "msl_get_wave": ([_vp, _i, _vp], _i),
+    "msl_set_detector": ([_vp, _i, C.c_uint64], _i), "msl_get_detector_image": ([_vp, _vp], _i),
+    "msl_readout_noise": ([_vp, C.c_longlong, C.POINTER(_f)], _i),
     "msl_server_create": ([C.POINTER(_vp), _i], _i), "msl_server_destroy": ([_vp], None),
     "msl_server_last_error": ([_vp], _cp),
     "msl_server_run": ([_vp, C.POINTER(msl_serve_params), C.POINTER(msl_job), _i, _vp], _i),

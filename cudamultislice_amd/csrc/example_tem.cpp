@@ -2,13 +2,14 @@
 // ("CUDA Simulation GUI/MainWindow.xaml.cs":238-283, slice count as
 // "GUI Simulation/ManagedMultislice.cpp":105-111) written against the drop-in
 // facade: load, bin, parameterise, microscope, sampling, plane wave,
-// conventional slices, exit wave, CTF image.
+// conventional slices, exit wave, CTF image through the detector model.
 //
 //   example_tem <atoms.xyz> <out-prefix> [N=256] [ps=0.084844] [kV=200] [dz=1]
+//               [dose=1000]
 //
 // Writes <prefix>_ew.bin (N*N complex64, row-major) and <prefix>_image.bin
-// (N*N float32) and prints one summary line.  Exit status 0 only if every call
-// succeeded.
+// (N*N float32: the detector image |SimulatedImage|, GetDQEImValue) and prints
+// one summary line.  Exit status 0 only if every call succeeded.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,7 +26,7 @@ static bool ok(const UnmanagedMultisliceSimulation& s, const char* what) {
 
 int main(int argc, char** argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s atoms.xyz out-prefix [N] [ps] [kV] [dz]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s atoms.xyz out-prefix [N] [ps] [kV] [dz] [dose]\n", argv[0]);
     return 2;
   }
   const std::string path = argv[1], prefix = argv[2];
@@ -33,6 +34,7 @@ int main(int argc, char** argv) {
   const float ps = argc > 4 ? (float)std::atof(argv[4]) : 0.084844f;
   const float kV = argc > 5 ? (float)std::atof(argv[5]) : 200.f;
   const float dz = argc > 6 ? (float)std::atof(argv[6]) : 1.f;
+  const float dose = argc > 7 ? (float)std::atof(argv[7]) : 1000.f;
 
   UnmanagedMultisliceSimulation sim;
   if (!ok(sim, "create")) return 1;
@@ -57,11 +59,11 @@ int main(int argc, char** argv) {
   sim.GetExitWave();
   if (!ok(sim, "GetExitWave")) return 1;
   sim.AllocHostImage();
-  sim.SimulateImage(1.f);
+  sim.SimulateImage(dose);  // electrons per pixel
   if (!ok(sim, "SimulateImage")) return 1;
 
   std::vector<float> img((size_t)N * N);
-  for (size_t i = 0; i < img.size(); ++i) img[i] = sim.GetImValue((int)(i % N), (int)(i / N));
+  for (size_t i = 0; i < img.size(); ++i) img[i] = sim.GetDQEImValue((int)(i % N), (int)(i / N));
   FILE* f = std::fopen((prefix + "_ew.bin").c_str(), "wb");
   FILE* g = std::fopen((prefix + "_image.bin").c_str(), "wb");
   const bool wrote = f && g && std::fwrite(sim.EW, sizeof(msl_float2), (size_t)N * N, f) == (size_t)N * N &&
@@ -72,8 +74,9 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "could not write %s_*.bin\n", prefix.c_str());
     return 1;
   }
-  std::printf("slices %d dz %g |EW| max %g min %g at (10,20) %g image contrast %g\n", n, sim.conventionaldz,
-              sim.maxposEW, sim.minposEW, sim.GetEWValueAbs(10, 20), sim.imagecontrast);
+  std::printf("slices %d dz %g |EW| max %g min %g at (10,20) %g image contrast %g read (10,20) %g\n", n,
+              sim.conventionaldz, sim.maxposEW, sim.minposEW, sim.GetEWValueAbs(10, 20), sim.imagecontrast,
+              sim.GetImValue(10, 20));
   sim.FreeExitWave();
   return 0;
 }

@@ -8,8 +8,11 @@
 UnmanagedMultisliceSimulation::UnmanagedMultisliceSimulation()
     : MaximumX(0), MaximumY(0), MaximumZ(0), MinimumX(0), MinimumY(0), MinimumZ(0), numberOfSlices(0), kmax(0),
       conventionaldz(1), maxposIm(0), maxposEW(0), minposEW(0), minposIm(0), EW(nullptr), TDS(nullptr),
-      TDS2(nullptr), SimulatedImage(nullptr), imagecontrast(0), sim_(nullptr), status_(MSL_OK), resX_(0), resY_(0) {
+      TDS2(nullptr), SimulatedImage(nullptr), imagecontrast(0), sim_(nullptr), status_(MSL_OK), resX_(0), resY_(0),
+      reads_(0) {
   status_ = msl_create(&sim_, 0);
+  // SimulateImage runs the reference's detector model (seeded noise)
+  if (status_ == MSL_OK) status_ = msl_set_detector(sim_, 1, 0x4D534CULL);
 }
 
 UnmanagedMultisliceSimulation::~UnmanagedMultisliceSimulation() {
@@ -80,15 +83,20 @@ void UnmanagedMultisliceSimulation::AllocHostImage() {
   delete[] SimulatedImage;
   SimulatedImage = new msl_float2[(size_t)resX_ * resY_]();
 }
-// deterministic CTF image; SimulatedImage holds (I, 0) per pixel
+// CTF image followed by the detector model (U.cu:2322-2428): SimulatedImage
+// holds the complex detector image; with the model switched off it holds
+// (I, 0) of the CTF intensity
 void UnmanagedMultisliceSimulation::SimulateImage(float dose) {
   msl_status st = msl_simulate_image(sim_, dose);
   if (st == MSL_OK) {
     if (!SimulatedImage) AllocHostImage();
-    float* tmp = new float[(size_t)resX_ * resY_];
-    st = msl_get_image(sim_, tmp);
-    for (size_t i = 0; i < (size_t)resX_ * resY_; ++i) SimulatedImage[i] = msl_float2{tmp[i], 0.f};
-    delete[] tmp;
+    st = msl_get_detector_image(sim_, reinterpret_cast<float*>(SimulatedImage));
+    if (st == MSL_ERR_STATE) {  // detector model off
+      float* tmp = new float[(size_t)resX_ * resY_];
+      st = msl_get_image(sim_, tmp);
+      for (size_t i = 0; i < (size_t)resX_ * resY_; ++i) SimulatedImage[i] = msl_float2{tmp[i], 0.f};
+      delete[] tmp;
+    }
   }
   Sync(st);
 }
@@ -117,13 +125,17 @@ float UnmanagedMultisliceSimulation::GetEWValueAbs(int x, int y) {
 }
 float UnmanagedMultisliceSimulation::GetEWValueRe(int x, int y) { return EW[x + y * resX_].x; }
 float UnmanagedMultisliceSimulation::GetEWValueIm(int x, int y) { return EW[x + y * resX_].y; }
-// CTF image value (the reference adds readout noise of its detector model,
-// U.cu:2497-2512, which this engine does not simulate)
+// |SimulatedImage| plus readout noise, a fresh draw per call (U.cu:2497-2508)
 float UnmanagedMultisliceSimulation::GetImValue(int x, int y) {
+  float noise = 4.f;
+  msl_readout_noise(sim_, reads_++, &noise);
+  return noise + GetDQEImValue(x, y);
+}
+// |SimulatedImage| (U.cu:2510-2512)
+float UnmanagedMultisliceSimulation::GetDQEImValue(int x, int y) {
   const msl_float2 v = SimulatedImage[x + y * resX_];
   return std::sqrt(v.x * v.x + v.y * v.y);
 }
-float UnmanagedMultisliceSimulation::GetDQEImValue(int x, int y) { return GetImValue(x, y); }
 float UnmanagedMultisliceSimulation::GetTDSMin() {
   float m = FLT_MAX;
   for (size_t j = 0; j < (size_t)resX_ * resY_; ++j) m = TDS[j] < m ? TDS[j] : m;

@@ -37,6 +37,8 @@ static int lg2(int n) {
 //         2 as 0 with the alternative plan
 //         3 as 0 with half as many columns per workgroup (more workgroups
 //           per CU to overlap the barriers)
+//         4 as 0 with two LDS copies used by alternate exchanges (one
+//           barrier per exchange instead of two)
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 2 GiB of
@@ -49,7 +51,7 @@ void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 3 : 4; }
+int num_pass_variants(int kind) { return kind == 0 ? 3 : 5; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
@@ -63,8 +65,11 @@ template <int N, int CV> struct ColCfg {
   // start two banks apart (stride = 2 mod 32 float2)
   static constexpr int SPAN = LdsLine<SH>::span(N);
   static constexpr int LSC = SPAN + ((2 - SPAN % 32) + 32) % 32;
-  static constexpr int LDS = INTERLEAVED ? N * C : C * LSC;
+  static constexpr bool DUAL = CV == 4;
+  static constexpr int LDS = INTERLEAVED ? N * C * (DUAL ? 2 : 1) : C * LSC;
   static constexpr int THREADS = C * T;
+  // no occupancy floor: forcing 4 waves per SIMD spills the long transforms
+  static constexpr int MIN_WAVES = 1;
 };
 
 int col_tile(int ny) {
@@ -233,7 +238,7 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, co
 // variants 0, 1: the C columns of a tile spread over the workgroup, thread
 // (col, t) holds rows t + c T of column col; workgroup barriers
 template <int N, int MODE, int MUL, int CV>
-__global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
+__global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_WAVES)) k_col(ColArgs a) {
   using Cfg = ColCfg<N, CV>;
   constexpr int T = Cfg::T, E = N / T, C = Cfg::C;
   __shared__ float2 lds[Cfg::LDS];
@@ -252,7 +257,12 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS)) k_col(ColArgs a) {
   }
   using LL = LdsLine<Cfg::SH>;
   const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
-  col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, L);
+  if constexpr (Cfg::DUAL) {
+    const DualLine<LL> D{L, LL{lds + N * C + col, C}};
+    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, D);
+  } else {
+    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, L);
+  }
   if constexpr (MODE == COL_FWD_DETECT) {
     const float kxv = a.kx[x];
     float acc = 0.f;
@@ -361,7 +371,11 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
 template <int N>
 static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
   // the detector pass reduces over the workgroup in an interleaved layout
-  if (mode == COL_FWD_DETECT) return var == 2 ? col_nv<N, 2>(mode, mul, a, s) : col_nv<N, 0>(mode, mul, a, s);
+  if (mode == COL_FWD_DETECT) {
+    if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
+    return var == 2 ? col_nv<N, 2>(mode, mul, a, s) : col_nv<N, 0>(mode, mul, a, s);
+  }
+  if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
   if (var == 3) return col_nv<N, 3>(mode, mul, a, s);
   if (var == 2) return col_nv<N, 2>(mode, mul, a, s);
   if (var == 1) return col_nv<N, 1>(mode, mul, a, s);

@@ -539,6 +539,77 @@ hipError_t launch_intensity(const float2* w, float* o, long long n, float f, hip
   return hipGetLastError();
 }
 
+// ============================================================== detector model
+// Radial curve of DQEFunc (U.cu:1120-1132): signed frequency indices, radius
+// sqrtf(x'^2 + y'^2) / binning in FP32, entry floor(radius) up to 724, else
+// 0.0001.  F *= scale * (root ? sqrtf(curve) : curve)  (DQEKernel takes the
+// square root, NTFKernel the curve itself, U.cu:1134-1186).
+__global__ void k_detector_filter(float2* F, int nx, int ny, const float* tab, int binning, int root, float scale) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= nx) return;
+  const int xd = x < nx / 2 ? x : x - nx, yd = y < ny / 2 ? y : y - ny;
+  const float r = sqrtf((float)(xd * xd + yd * yd)) / (float)binning;
+  const int i = (int)floorf(r);
+  const float v = i <= 724 ? tab[i] : 0.0001f;
+  const float f = (root ? sqrtf(v) : v) * scale;
+  float2& e = F[(long long)y * nx + x];
+  e = make_float2(e.x * f, e.y * f);
+}
+hipError_t launch_detector_filter(float2* F, int nx, int ny, const float* tab, int binning, int root, float scale,
+                                  hipStream_t s) {
+  hipLaunchKernelGGL(k_detector_filter, dim3((nx + 255) / 256, ny), dim3(256), 0, s, F, nx, ny, tab, binning, root,
+                     scale);
+  return hipGetLastError();
+}
+
+// Philox4x64-10 (Salmon et al., SC'11) on the device; the same generator as
+// the host's frozen-phonon displacements.
+__device__ void philox4x64_dev(unsigned long long c[4], unsigned long long k0, unsigned long long k1) {
+  const unsigned long long M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+  const unsigned long long W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
+    const unsigned long long hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
+    const unsigned long long n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += W0; k1 += W1;
+  }
+}
+
+// Shot noise of the detector (U.cu:2375-2396), seeded: a = |z| zscale,
+// m = dose a, counts = round(m + sqrt(|m|) g) with g the Box-Muller normal of
+// Philox block (i, 0xDE7EC7, 3, 0) under key (seed, 0x4D534C), clamped to
+// +-3; out = (counts, 0).
+__global__ void k_shot_noise(const float2* z, float2* out, long long n, float zscale, float dose,
+                             unsigned long long seed) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned long long c[4] = {(unsigned long long)i, 0xDE7EC7ULL, 3ULL, 0ULL};
+  philox4x64_dev(c, seed, 0x4D534CULL);
+  const double u1 = ((double)(c[0] >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+  const double u2 = (double)(c[1] >> 11) * (1.0 / 9007199254740992.0);
+  double g = sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+  g = g < -3.0 ? -3.0 : (g > 3.0 ? 3.0 : g);
+  const float2 v = z[i];
+  const double a = (double)(sqrtf(v.x * v.x + v.y * v.y) * zscale);
+  const double m = (double)dose * a;
+  out[i] = make_float2((float)round(m + sqrt(fabs(m)) * g), 0.f);
+}
+hipError_t launch_shot_noise(const float2* z, float2* out, long long n, float zscale, float dose,
+                             unsigned long long seed, hipStream_t s) {
+  hipLaunchKernelGGL(k_shot_noise, dim3(nblk(n, 256)), dim3(256), 0, s, z, out, n, zscale, dose, seed);
+  return hipGetLastError();
+}
+
+__global__ void k_real_to_complex(const float* in, float2* out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(in[i], 0.f);
+}
+hipError_t launch_real_to_complex(const float* in, float2* out, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_real_to_complex, dim3(nblk(n, 256)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
 // ============================================================== CTF
 // imagingKernel2, U.cu:1044-1080.  chi and the envelope arguments are
 // evaluated in FP64 (they reach ~1e2 rad at large Cs).

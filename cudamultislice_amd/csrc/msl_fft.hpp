@@ -142,11 +142,25 @@ struct WaveSync {
 // pad(p) = p + (p >> SH) (SH = 0: unpadded), which spreads the power-of-two
 // strides of the Stockham scatter over the LDS banks.
 template <int SH> struct LdsLine {
+  static constexpr bool kDual = false;
   float2* base;
   int stride;
   static __host__ __device__ constexpr int pad(int p) { return SH > 0 ? p + (p >> SH) : p; }
   static __host__ __device__ constexpr int span(int n) { return pad(n - 1) + 1; }
   __device__ __forceinline__ float2& at(int p) const { return base[pad(p) * stride]; }
+  template <int B> __device__ __forceinline__ const LdsLine& get() const { return *this; }
+};
+
+// Two copies of a line's LDS storage used by alternate exchanges: a stage
+// then needs one barrier (before its reads) instead of two, because a write
+// to one copy can only meet reads of the exchange two back, which every
+// thread finished before the previous exchange's barrier.  fft_line picks the
+// copy per stage so that consecutive exchanges alternate, also across the
+// two transforms of a pass (which run in opposite directions).
+template <class LL> struct DualLine {
+  static constexpr bool kDual = true;
+  LL a, b;
+  template <int B> __device__ __forceinline__ const LL& get() const { return B ? b : a; }
 };
 
 // One Stockham stage of radix R after L points have been combined.  Input:
@@ -154,7 +168,7 @@ template <int SH> struct LdsLine {
 // x[j + m N/R] = v[b + m NB]; outputs go to y[(j/L) L R + (j%L) + m L].
 // If LAST, outputs stay in registers in canonical order; otherwise they are
 // written to LDS, and read back in canonical order after a barrier.
-template <int N, int T, int R, int L, int DIR, bool LAST, class S, class LL>
+template <int N, int T, int R, int L, int DIR, bool LAST, class S, int BUF, class LL>
 __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const float2* __restrict__ tw,
                                                const LL& lds) {
   constexpr int E = N / T, NB = E / R;
@@ -183,7 +197,7 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
     } else {
       const int base = (j / L) * L * R + k;
 #pragma unroll
-      for (int m = 0; m < R; ++m) lds.at(base + m * L) = a[m];
+      for (int m = 0; m < R; ++m) lds.template get<BUF>().at(base + m * L) = a[m];
     }
   }
   if constexpr (LAST) {
@@ -192,8 +206,8 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
   } else {
     S::sync();
 #pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = lds.at(t + c * T);
-    S::sync();
+    for (int c = 0; c < E; ++c) v[c] = lds.template get<BUF>().at(t + c * T);
+    if constexpr (!LL::kDual) S::sync();
   }
 }
 
@@ -203,13 +217,13 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
 template <class P, int N, int DIR, class S = BlockSync, class LL = LdsLine<0>>
 __device__ __forceinline__ void fft_line(float2 (&v)[N / P::T], int t, const float2* __restrict__ tw, const LL& lds) {
   constexpr int T = P::T;
-  if constexpr (P::R3 > 1) {
-    stockham_stage<N, T, P::R1, 1, DIR, false, S>(v, t, tw, lds);
-    stockham_stage<N, T, P::R2, P::R1, DIR, false, S>(v, t, tw, lds);
-    stockham_stage<N, T, P::R3, P::R1 * P::R2, DIR, true, S>(v, t, tw, lds);
-  } else {
-    stockham_stage<N, T, P::R1, 1, DIR, false, S>(v, t, tw, lds);
-    stockham_stage<N, T, P::R2, P::R1, DIR, true, S>(v, t, tw, lds);
+  if constexpr (P::R3 > 1) {  // two exchanges: copies 0, 1 in every transform
+    stockham_stage<N, T, P::R1, 1, DIR, false, S, 0>(v, t, tw, lds);
+    stockham_stage<N, T, P::R2, P::R1, DIR, false, S, 1>(v, t, tw, lds);
+    stockham_stage<N, T, P::R3, P::R1 * P::R2, DIR, true, S, 0>(v, t, tw, lds);
+  } else {  // one exchange: the copy follows the direction
+    stockham_stage<N, T, P::R1, 1, DIR, false, S, (DIR < 0 ? 1 : 0)>(v, t, tw, lds);
+    stockham_stage<N, T, P::R2, P::R1, DIR, true, S, 0>(v, t, tw, lds);
   }
 }
 

@@ -860,17 +860,50 @@ void ctf_images(msl_sim* s, int ndf, const float* dfs, float* d_out) {
   HIPC(launch_intensity(K.p, d_out, n * ndf, 1.0f, s->stream));
   HIPC(hipStreamSynchronize(s->stream));
 }
+
+// Detector model of SimulateImage (U.cu:2345-2428) on the CTF intensity in
+// s->image: FFT nf, sqrt(DQE) filter, IFFT nf, modulus, seeded shot noise
+// with rounding to counts, FFT nf, NTF filter, IFFT times 64 nf (8 CCD
+// counts per electron, squared, U.cu:2360, 2423).  Result in s->detimg.
+void detector_image(msl_sim* s, float dose) {
+  const long long n = (long long)s->npix();
+  if (!s->dqe_tab.p) {
+    s->dqe_tab.ensure(725);
+    s->ntf_tab.ensure(725);
+    HIPC(hipMemcpyAsync(s->dqe_tab.p, kDetectorDQE, sizeof(kDetectorDQE), hipMemcpyHostToDevice, s->stream));
+    HIPC(hipMemcpyAsync(s->ntf_tab.p, kDetectorNTF, sizeof(kDetectorNTF), hipMemcpyHostToDevice, s->stream));
+  }
+  s->detimg.ensure(n);
+  float2* W = s->B.p;
+  const int binning = 4;  // DQEKernel / NTFKernel launches, U.cu:2367, 2421
+  HIPC(launch_real_to_complex(s->image.p, W, n, s->stream));
+  fft2(s, W, 1, false);
+  HIPC(launch_detector_filter(W, s->resX, s->resY, s->dqe_tab.p, binning, 1, s->nf, s->stream));
+  fft2(s, W, 1, true);
+  HIPC(launch_shot_noise(W, s->detimg.p, n, s->nf, dose, s->detector_seed, s->stream));
+  fft2(s, s->detimg.p, 1, false);
+  HIPC(launch_detector_filter(s->detimg.p, s->resX, s->resY, s->ntf_tab.p, binning, 0, s->nf, s->stream));
+  fft2(s, s->detimg.p, 1, true, 64.f * s->nf);
+  s->have_detimg = true;
+}
 }  // namespace
 
 msl_status msl_simulate_image(msl_sim* s, float dose) {
   API_BEGIN(s)
-  (void)dose;
   require(s->have_waves, MSL_ERR_STATE, "no wave");
   s->image.ensure(s->npix());
   const float df = s->df;
   ctf_images(s, 1, &df, s->image.p);
+  s->have_detimg = false;
   std::vector<float> im(s->npix());
-  d2h(s, im.data(), s->image.p, im.size() * sizeof(float));
+  if (s->detector) {
+    detector_image(s, dose);
+    std::vector<float2> z(s->npix());
+    d2h(s, z.data(), s->detimg.p, z.size() * sizeof(float2));
+    for (size_t i = 0; i < z.size(); ++i) im[i] = std::sqrt(z[i].x * z[i].x + z[i].y * z[i].y);
+  } else {
+    d2h(s, im.data(), s->image.p, im.size() * sizeof(float));
+  }
   // display range and contrast (U.cu:2430-2491) on the CTF intensity:
   // standard deviation over the top-left min(512, N) square
   size_t imx = 0, imn = 0;
@@ -893,6 +926,32 @@ msl_status msl_simulate_image(msl_sim* s, float dose) {
   s->imagecontrast = (float)std::sqrt(var / ((double)w * h));
   s->have_image = true;
   API_END(s)
+}
+
+msl_status msl_set_detector(msl_sim* s, int enabled, uint64_t seed) {
+  if (!s) return MSL_ERR_ARG;
+  s->detector = enabled != 0;
+  s->detector_seed = seed;
+  s->have_detimg = false;
+  return MSL_OK;
+}
+
+msl_status msl_get_detector_image(msl_sim* s, float* out) {
+  API_BEGIN(s)
+  require(s->have_detimg, MSL_ERR_STATE, "SimulateImage with the detector model first");
+  require(out != nullptr, MSL_ERR_ARG, "null output");
+  d2h(s, out, s->detimg.p, s->npix() * sizeof(float2));
+  API_END(s)
+}
+
+// GetImValue's readout noise, U.cu:2497-2512: 4 + 1.33 g with g a standard
+// normal clamped to +-3 (draw `counter` of the detector's Philox stream)
+msl_status msl_readout_noise(const msl_sim* s, long long counter, float* out) {
+  if (!s || !out || counter < 0) return MSL_ERR_ARG;
+  double g = philox_normal(s->detector_seed, 0xDE7EC7ULL, (uint64_t)counter, 4);
+  g = g < -3 ? -3 : (g > 3 ? 3 : g);
+  *out = (float)(4.0 + g * 1.33);
+  return MSL_OK;
 }
 
 msl_status msl_get_image(msl_sim* s, float* out) {

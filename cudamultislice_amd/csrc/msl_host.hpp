@@ -11,6 +11,7 @@
 
 #include "../../include/msl.h"
 #include "fparams_table.h"
+#include "mtf_tables.h"
 #include "msl_internal.hpp"
 
 namespace mslh {
@@ -106,6 +107,12 @@ struct msl_sim {
   mslh::DBuf<float2> PsiMinus, Psi, PsiPlus, V, G, B;
   bool have_waves = false, fd_ready = false;
   mslh::DBuf<float> tds, image, red;
+  // stochastic detector model (SimulateImage, U.cu:2345-2428)
+  bool detector = false;
+  uint64_t detector_seed = 0x4D534CULL;
+  mslh::DBuf<float> dqe_tab, ntf_tab;
+  mslh::DBuf<float2> detimg;
+  bool have_detimg = false;
   bool have_tds = false, have_image = false;
   float maxposIm = 0, maxposEW = 0, minposEW = 0, minposIm = 0, imagecontrast = 0;
   float timings[4] = {0, 0, 0, 0};

@@ -160,6 +160,13 @@ struct CtfArgs {
   float Cs, A1, A1phi, A2, A2phi, obj, wavel, beta, delta, scale;
 };
 hipError_t launch_ctf(const CtfArgs& a, hipStream_t s);
+
+// stochastic detector model (SimulateImage, U.cu:2345-2428)
+hipError_t launch_detector_filter(float2* F, int nx, int ny, const float* tab, int binning, int root, float scale,
+                                  hipStream_t s);
+hipError_t launch_shot_noise(const float2* z, float2* out, long long n, float zscale, float dose,
+                             unsigned long long seed, hipStream_t s);
+hipError_t launch_real_to_complex(const float* in, float2* out, long long n, hipStream_t s);
 hipError_t launch_intensity(const float2* w, float* out, long long n, float s, hipStream_t st);
 
 // TDS / CBED: tds[y][x] += |F[(y+ny/2)%ny][(x+nx/2)%nx]| (U.cu:2734-2766)

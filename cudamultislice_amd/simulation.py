@@ -142,10 +142,26 @@ class MultisliceSimulation:
         return o
 
     def SimulateImage(self, dose: float = 1.0):
+        """Runs SimulateImage; returns the CTF intensity (the detector model's
+        output, when enabled, is read by detector_image())."""
         self._c(lib().msl_simulate_image(self._h, dose))
         o = np.zeros((self.resY, self.resX), np.float32)
         self._c(lib().msl_get_image(self._h, _p(o)))
         return o
+
+    def set_detector(self, enabled: bool = True, seed: int = 0x4D534C):
+        """Stochastic K2 detector model in SimulateImage (U.cu:2345-2428)."""
+        self._c(lib().msl_set_detector(self._h, 1 if enabled else 0, seed))
+
+    def detector_image(self):
+        o = self._cplx()
+        self._c(lib().msl_get_detector_image(self._h, _p(o)))
+        return o
+
+    def readout_noise(self, counter: int) -> float:
+        v = C.c_float()
+        self._c(lib().msl_readout_noise(self._h, counter, C.byref(v)))
+        return v.value
 
     def ctf_sweep(self, dfs):
         dfs = np.ascontiguousarray(dfs, np.float32)

@@ -82,11 +82,12 @@ class UnmanagedMultisliceSimulation {
   // additions (no counterpart in the reference)
   msl_status GetLastStatus() const { return status_; }
   const char* GetLastError() const;
-  msl_sim* Handle() { return sim_; }
+  msl_sim* Handle() { return sim_; }  // e.g. msl_set_detector(Handle(), 0, seed) for the CTF image only
 
  private:
   void Sync(msl_status st);
   msl_sim* sim_;
   msl_status status_;
   int resX_, resY_;
+  long long reads_;  // readout-noise draws taken by GetImValue
 };

@@ -119,10 +119,23 @@ msl_status msl_get_stem_exit_wave(msl_sim* sim, float* out);
 msl_status msl_alloc_tds(msl_sim* sim);
 msl_status msl_add_tds_waves(msl_sim* sim);
 msl_status msl_get_tds(msl_sim* sim, float* out);
-/* deterministic part of SimulateImage (U.cu:2322-2343): the CTF image
- * |IFFT(CTF . FFT(psi))|^2, stored for msl_get_image */
+/* SimulateImage (U.cu:2322-2428): the CTF image |IFFT(CTF . FFT(psi))|^2,
+ * stored for msl_get_image; with the detector model enabled it continues
+ * with the reference's K2 detector: DQE filter (sqrt of the curve), shot
+ * noise round(dose a + sqrt(dose a) g) with g a seeded clamped normal
+ * (Philox instead of rand()), NTF filter, 8 counts per electron; the complex
+ * result is read by msl_get_detector_image and maxposIm / minposIm /
+ * imagecontrast describe its modulus */
 msl_status msl_simulate_image(msl_sim* sim, float dose);
 msl_status msl_get_image(msl_sim* sim, float* out);
+/* detector model on / off (off by default: msl_simulate_image gives the CTF
+ * image only) and the seed of its noise */
+msl_status msl_set_detector(msl_sim* sim, int enabled, uint64_t seed);
+/* SimulatedImage of the reference: [resY][resX][2] */
+msl_status msl_get_detector_image(msl_sim* sim, float* out);
+/* readout noise of GetImValue (U.cu:2497-2512): 4 + 1.33 g, g the clamped
+ * normal of draw `counter` of the detector's noise stream */
+msl_status msl_readout_noise(const msl_sim* sim, long long counter, float* out);
 /* CTF images for several defoci from the current wave: out is ndf*resX*resY */
 msl_status msl_ctf_sweep(msl_sim* sim, int ndf, const float* dfs, float* out);
 /* annular dark-field signal of the current wave: sum of |FFT_unitary(psi)|^2
@@ -193,7 +206,7 @@ msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, flo
  * k_pass.hip.  msl_bench_pass times `reps` launches on an n x n batch of
  * nbatch waves and, with maxdiff, compares against variant 0;
  * msl_autotune_passes selects the fastest agreeing variant per pass for n
- * (out[8]: [0..3] row and [4..7] column timings in ms per variant, -1 where
+ * (out[10]: [0..4] row and [5..9] column timings in ms per variant, -1 where
  * rejected or absent).  Timings use band-pruned passes where the length allows. */
 msl_status msl_set_pass_variant(int kind, int n, int variant);
 msl_status msl_get_pass_variant(int kind, int n, int* variant);

@@ -383,6 +383,20 @@ void orc_fft2(int ny, int nx, double* data, int inverse) {
 
 void orc_philox4x64(const uint64_t* ctr, const uint64_t* key, uint64_t* out) { philox4x64(ctr, key, out); }
 
+// detector model of SimulateImage (U.cu:2345-2428) on an intensity image
+void orc_detector_image(int nx, int ny, const double* I, float dose, uint64_t seed, const float* dqe,
+                        const float* ntf, int binning, double* out, double* stats) {
+  std::vector<double> in(I, I + (size_t)nx * ny);
+  std::vector<std::complex<double>> o;
+  detector_image<double>(nx, ny, in, dose, seed, dqe, ntf, binning, o, stats);
+  from_complex(o, out);
+}
+float orc_detector_table(const float* tab, int x, int y, int nx, int ny, int binning) {
+  return detector_table(tab, x, y, nx, ny, binning);
+}
+double orc_detector_normal(uint64_t seed, uint64_t i) { return detector_normal(seed, i); }
+float orc_readout_noise(uint64_t seed, uint64_t counter) { return readout_noise(seed, counter); }
+
 // Frozen-phonon displacements for one configuration, U.cu:1549-1551 with the
 // Philox/Box-Muller generator of msl_oracle.hpp.
 void orc_tds_displacements(uint64_t seed, int config, int n, float sigma, float* dx, float* dy, float* dz) {

@@ -231,6 +231,85 @@ inline double philox_normal(uint64_t seed, uint64_t config, uint64_t atom, uint6
   return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
 }
 
+// ---------------------------------------------------------------- detector
+// Radial table lookup of DQEFunc (U.cu:1120-1132): signed frequency indices
+// (x - N for x >= N/2), r = sqrtf(x'^2 + y'^2) / binning in FP32, entry
+// floor(r) up to 724, else 0.0001.
+inline float detector_table(const float* tab, int x, int y, int nx, int ny, int binning) {
+  const int xd = x < nx / 2 ? x : x - nx, yd = y < ny / 2 ? y : y - ny;
+  volatile float r = std::sqrt((float)(xd * xd + yd * yd));
+  r = r / (float)binning;
+  const int i = (int)std::floor((float)r);
+  return i <= 724 ? tab[i] : 0.0001f;
+}
+
+// Standard normal of the detector's shot noise at pixel i (Philox block
+// (i, 0xDE7EC7, 3, 0) under key (seed, 0x4D534C)), clamped to +-3 as the
+// reference clamps its rand() Box-Muller draw (U.cu:2385-2392).
+inline double detector_normal(uint64_t seed, uint64_t i, uint64_t axis = 3) {
+  const double g = philox_normal(seed, 0xDE7EC7ULL, i, axis);
+  return g < -3 ? -3 : (g > 3 ? 3 : g);
+}
+
+// Readout noise of GetImValue (U.cu:2497-2508): 4 + 1.33 g, draw `counter`
+// (Philox block (counter, 0xDE7EC7, 4, 0)).
+inline float readout_noise(uint64_t seed, uint64_t counter) {
+  return (float)(4.0 + detector_normal(seed, counter, 4) * 1.33);
+}
+
+// Stochastic detector model of SimulateImage (U.cu:2345-2428): the
+// intensity image I -> FFT nf -> * sqrt(DQE(r)) -> IFFT nf -> modulus a ->
+// counts round(dose a + sqrt(|dose a|) g) -> FFT nf -> * NTF(r) ->
+// IFFT (64 nf) (8 CCD counts per electron, squared).  out: the complex
+// image; stats: |out| at its first maximum and minimum, and the standard
+// deviation of |out| over the top-left min(N, 512)^2 pixels (the
+// reference's contrast region, U.cu:2454-2491).
+template <class Real>
+void detector_image(int nx, int ny, const std::vector<Real>& I, float dose, uint64_t seed, const float* dqe,
+                    const float* ntf, int binning, std::vector<std::complex<Real>>& out, double stats[3]) {
+  using C = std::complex<Real>;
+  const size_t n = (size_t)nx * ny;
+  const Real nf = Real(1) / std::sqrt(Real(n));
+  FFT2D<Real> f;
+  f.init(nx, ny);
+  std::vector<C> F(n);
+  for (size_t i = 0; i < n; ++i) F[i] = C(I[i], 0);
+  f.run(F.data(), false);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x)
+      F[(size_t)y * nx + x] *= nf * std::sqrt(Real(detector_table(dqe, x, y, nx, ny, binning)));
+  f.run(F.data(), true);
+  for (size_t i = 0; i < n; ++i) {
+    const Real a = std::abs(F[i] * nf);
+    const Real m = Real(dose) * a;
+    F[i] = C(std::round(m + std::sqrt(std::fabs(m)) * Real(detector_normal(seed, i))), 0);
+  }
+  f.run(F.data(), false);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) F[(size_t)y * nx + x] *= nf * Real(detector_table(ntf, x, y, nx, ny, binning));
+  f.run(F.data(), true);
+  out.resize(n);
+  for (size_t i = 0; i < n; ++i) out[i] = F[i] * (Real(64) * nf);
+  size_t imx = 0, imn = 0;
+  for (size_t i = 1; i < n; ++i) {
+    if (std::abs(out[i]) > std::abs(out[imx])) imx = i;
+    if (std::abs(out[i]) < std::abs(out[imn])) imn = i;
+  }
+  stats[0] = (double)std::abs(out[imx]);
+  stats[1] = (double)std::abs(out[imn]);
+  const int sw = std::min(nx, 512), sh = std::min(ny, 512);
+  double mean = 0, var = 0;
+  for (int y = 0; y < sh; ++y)
+    for (int x = 0; x < sw; ++x) mean += (double)std::abs(out[(size_t)y * nx + x]);
+  mean /= (double)sw * sh;
+  for (int y = 0; y < sh; ++y)
+    for (int x = 0; x < sw; ++x) {
+      const double d = (double)std::abs(out[(size_t)y * nx + x]) - mean;
+      var += d * d;
+    }
+  stats[2] = std::sqrt(var / ((double)sw * sh));
+}
+
 // ---------------------------------------------------------------- element map
 // GetZNum, U.cu:1188-1243: the reference's 25-symbol table; any other symbol
 // (including Ti) maps to hydrogen.

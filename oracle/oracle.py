@@ -60,6 +60,10 @@ def _lib():
         "orc_vatom": ([i, d, _fp], d), "orc_vzatom": ([i, d, _fp], d),
         "orc_fft2": ([i, i, _dp, i], None),
         "orc_philox4x64": ([_up, _up, _up], None),
+        "orc_detector_image": ([i, i, _dp, f, C.c_uint64, _fp, _fp, i, _dp, _dp], None),
+        "orc_detector_table": ([_fp, i, i, i, i, i], f),
+        "orc_detector_normal": ([C.c_uint64, C.c_uint64], d),
+        "orc_readout_noise": ([C.c_uint64, C.c_uint64], f),
         "orc_tds_displacements": ([C.c_uint64, i, i, f, _fp, _fp, _fp], None),
         "orc_ref_znum": ([C.c_char_p], i),
     }
@@ -112,6 +116,30 @@ def philox4x64(ctr, key):
     out = np.zeros(4, np.uint64)
     L().orc_philox4x64(np.asarray(ctr, np.uint64), np.asarray(key, np.uint64), out)
     return out
+
+
+def detector_image(I, dose, seed, dqe, ntf, binning=4):
+    """Detector model of SimulateImage (U.cu:2345-2428) on an intensity image:
+    (complex image, [|max|, |min|, contrast])."""
+    I = np.ascontiguousarray(I, np.float64)
+    ny, nx = I.shape
+    out = np.zeros((ny, nx), np.complex128)
+    st = np.zeros(3)
+    L().orc_detector_image(nx, ny, I.ravel(), dose, seed, np.ascontiguousarray(dqe, np.float32),
+                           np.ascontiguousarray(ntf, np.float32), binning, out.view(np.float64).ravel(), st)
+    return out, st
+
+
+def detector_table(tab, x, y, nx, ny, binning=4):
+    return L().orc_detector_table(np.ascontiguousarray(tab, np.float32), x, y, nx, ny, binning)
+
+
+def detector_normal(seed, i):
+    return L().orc_detector_normal(seed, i)
+
+
+def readout_noise(seed, counter):
+    return L().orc_readout_noise(seed, counter)
 
 
 def tds_displacements(seed: int, config: int, n: int, sigma: float):

@@ -140,8 +140,8 @@ def test_example_tem_program_full_si_case_matches_oracle(fp, tmp_path):
         f.write(f"{Z.size}\n\n")
         for a, b, c in zip(x, y, z):
             f.write(f"Si {a:.6f} {b:.6f} {c:.6f}\n")
-    r = subprocess.run([exe, str(path), str(tmp_path / "out"), "256", "0.084844", "200", "1"], cwd=tmp_path,
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, str(path), str(tmp_path / "out"), "256", "0.084844", "200", "1", "1000"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     ew = np.fromfile(tmp_path / "out_ew.bin", np.complex64).reshape(256, 256)
     img = np.fromfile(tmp_path / "out_image.bin", np.float32).reshape(256, 256)
@@ -158,7 +158,47 @@ def test_example_tem_program_full_si_case_matches_oracle(fp, tmp_path):
     for it in range(1, n + 1):
         o.step_conv(it)
     assert rel_l2(ew, o.wave()) < 1e-4
-    assert rel_l2(img, o.ctf_image(-500.0)) < 1e-4
+    # the facade's SimulateImage runs the detector model (seed 0x4D534C, 1000
+    # electrons per pixel): integer counts, so FP32 vs FP64 inputs flip a few
+    # pixels by one count
+    from cudamultislice_amd.detector import tables
+    det, _ = O.detector_image(o.ctf_image(-500.0), 1000.0, 0x4D534C, *tables())
+    assert rel_l2(img, np.abs(det)) < 1e-3
+
+
+def test_tem_si_1024_252_slices_full_size_properties(fp):
+    """Si 14x14x92 cells (144 256 atoms), 1024^2 over 76.02 A, 2 A slices,
+    300 kV: at full size the exit wave is finite, the band-limited
+    propagation never gains intensity (|t| = 1, |P| <= 1), and the batched
+    8-defocus CTF sweep equals single-defocus images."""
+    Z, x, y, z = S.silicon(14, 14, 92)
+    from cudamultislice_amd import MultisliceSimulation
+    g = MultisliceSimulation()
+    g.set_atoms(Z, x, y, z)
+    g.GetParameterisation()
+    g.UploadBinnedAtoms()
+    g.ApplyMicroscopeParameters(300.0, -800.0, 0.0, 0.0, 1e4, 0.0, 0.0, 20.0)
+    inf = g.info()
+    g.SetCalculationVariables(0.07424, -800.0, 1024, 1024, inf["MaximumX"] - inf["MinimumX"],
+                              inf["MaximumY"] - inf["MinimumY"], inf["MaximumZ"] - inf["MinimumZ"], 2.0, 2.0)
+    g.PreCalculateFrequencies()
+    g.InitialiseWavefunctions(2.0)
+    n = g.GetSlices(True)
+    assert n == 252
+    norms = []
+    for it in range(1, n + 1):
+        g.MultisliceStepConv(it)
+        if it in (1, 60, 130, 200, n):
+            norms.append(float(np.mean(np.abs(g.get_wave(0)) ** 2)))
+    ew = g.GetExitWave()
+    assert np.all(np.isfinite(ew))
+    assert all(b <= a * (1 + 1e-5) for a, b in zip(norms, norms[1:])), norms
+    assert 0.2 < norms[-1] <= 1.0 + 1e-5
+    dfs = np.arange(-800, 601, 200, dtype=np.float32)
+    sweep = g.ctf_sweep(dfs)
+    single = g.SimulateImage(1.0)  # df -800 from ApplyMicroscopeParameters
+    assert np.all(np.isfinite(sweep)) and rel_l2(sweep[0], single) < 1e-6
+    assert len({round(float(s.std()), 6) for s in sweep}) == len(dfs)  # defocus changes the image
 
 
 def test_stem_probe_steps_and_tds_match_oracle(fp):
@@ -207,6 +247,39 @@ def test_ctf_image_matches_oracle(fp):
     sw = g.ctf_sweep(dfs)
     for k, d in enumerate(dfs):
         assert rel_l2(sw[k], o.ctf_image(float(d))) < 1e-4
+
+
+def test_detector_model_matches_oracle(fp):
+    """The K2 detector model of SimulateImage on the engine's own CTF image,
+    against the oracle's FP64 model with the same seed and curves; readout
+    noise draw by draw."""
+    from cudamultislice_amd.detector import tables
+    Z, x, y, z = _si_small()
+    g, o = _both(Z, x, y, z, fp)
+    g.UploadBinnedAtoms()
+    kw = dict(df=-500.0, Cs=1.2e7, obj=20.0)
+    _setup(g, o, 200.0, 0.2, 128, 20.0, off=(2.0, 2.0), **kw)
+    g.InitialiseWavefunctions(1.0)
+    for it in range(1, 9):
+        g.MultisliceStepConv(it)
+    dqe, ntf = tables()
+    for seed, dose in ((5, 300.0), (0x4D534C, 20.0)):
+        g.set_detector(True, seed)
+        I = g.SimulateImage(dose).astype(np.float64)  # the CTF intensity the model starts from
+        det = g.detector_image()
+        want, st = O.detector_image(I, dose, seed, dqe, ntf)
+        # integer counts: FP32 vs FP64 rounding of dose * a flips rare pixels
+        assert rel_l2(det, want) < 1e-4, (seed, dose)
+        inf = g.info()
+        assert inf["maxposIm"] == pytest.approx(st[0], rel=1e-3)
+        assert inf["imagecontrast"] == pytest.approx(st[2], rel=1e-3)
+    for k in (0, 1, 77):
+        assert g.readout_noise(k) == O.readout_noise(0x4D534C, k)
+    g.set_detector(False)
+    g.SimulateImage(1.0)
+    import ctypes as C
+    from cudamultislice_amd._lib import lib
+    assert lib().msl_get_detector_image(g._h, None) != 0  # model off: no detector image
 
 
 def test_fd_steps_match_oracle(fp):
@@ -295,7 +368,7 @@ def test_every_pass_variant_matches_oracle_in_batched_stem(fp):
     o.bin()
     want = o.stem_batch(px, py, dz, ns, 10.0, 200.0)["adf"]
     L = lib()
-    nrow, ncol = 3, 4
+    nrow, ncol = 3, 5
     try:
         for rv in range(nrow):
             for cv in range(ncol):
@@ -443,7 +516,23 @@ def test_diff_forward_and_gradient_match_fp64_adjoint():
     Io, _ = o.diff(V, psi0.astype(np.complex128), None, dz)
     assert rel_l2(I.detach().cpu().numpy(), Io) < 1e-4
     _, go = o.diff(V, psi0.astype(np.complex128), 2 * (Io - target), dz)
-    assert rel_l2(Vt.grad.cpu().numpy(), go) < 1e-3
+    g = Vt.grad.cpu().numpy()
+    assert rel_l2(g, go) < 1e-3
+    # central finite differences of the FP64 loss at 32 random voxels
+    pc = psi0.astype(np.complex128)
+    idx = [tuple(int(rng.integers(0, n)) for n in V.shape) for _ in range(32)]
+    fd, gv = [], []
+    for ix in idx:
+        Vp, Vm = V.copy(), V.copy()
+        Vp[ix] += np.float32(1e-2)
+        Vm[ix] -= np.float32(1e-2)
+        Lp = float(((o.diff(Vp, pc, None, dz)[0] - target) ** 2).sum())
+        Lm = float(((o.diff(Vm, pc, None, dz)[0] - target) ** 2).sum())
+        fd.append((Lp - Lm) / (float(Vp[ix]) - float(Vm[ix])))
+        gv.append(g[ix])
+    fd, gv = np.array(fd), np.array(gv)
+    assert np.abs(fd).max() > 0
+    assert np.linalg.norm(gv - fd) / np.linalg.norm(fd) < 1e-3
 
 
 def test_diff_checkpoint_interval_does_not_change_gradient():

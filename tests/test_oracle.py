@@ -521,3 +521,69 @@ def test_jittered_atoms_keep_min_distance_and_fill_box():
     # spread: every eighth of the box holds atoms
     occ = np.histogramdd(p, bins=(2, 2, 2), range=((0, 30), (0, 25), (0, 40)))[0]
     assert occ.min() > 0.08 * len(p)
+
+
+# ---------------------------------------------------------------- detector model
+def test_detector_table_lookup_uses_signed_binned_radius():
+    from cudamultislice_amd.detector import tables
+    dqe, ntf = tables()
+    assert dqe.size == ntf.size == 725 and dqe[0] == np.float32(0.72963) and ntf[0] == np.float32(1.14689)
+    N = 256
+    for k in (0, 1, 5, 31):
+        assert O.detector_table(dqe, 4 * k, 0, N, N) == dqe[k]       # r = 4k / 4
+        assert O.detector_table(dqe, 0, N - 4 * k if k else 0, N, N) == dqe[k]  # negative frequency
+        assert O.detector_table(ntf, 4 * k + 3, 0, N, N) == ntf[k]   # floor
+    # the index N/2 is the negative Nyquist: |-N/2| / 4
+    assert O.detector_table(dqe, N // 2, 0, N, N) == dqe[N // 8]
+    # 3-4-5: sqrt(12^2 + 16^2) / 4 = 5
+    assert O.detector_table(dqe, 12, N - 16, N, N) == dqe[5]
+
+
+def test_detector_normal_is_a_clamped_standard_normal():
+    from scipy.stats import norm
+    g = np.array([O.detector_normal(7, i) for i in range(100000)])
+    assert g.min() >= -3 and g.max() <= 3 and np.sum(np.abs(g) == 3) > 0
+    q = norm.sf(3)
+    var = (1 - 2 * q) - 2 * 3 * norm.pdf(3) + 2 * 9 * q  # E[clamp(g, -3, 3)^2]
+    assert abs(g.mean()) < 0.01 and abs(g.var() - var) < 0.01
+    assert O.detector_normal(7, 5) == O.detector_normal(7, 5) != O.detector_normal(8, 5)
+
+
+def test_detector_constant_image_mean_is_the_dc_gain():
+    from cudamultislice_amd.detector import tables
+    dqe, ntf = tables()
+    N, I0, dose = 128, 0.8, 50.0
+    out, st = O.detector_image(np.full((N, N), I0), dose, 3, dqe, ntf)
+    want = 64 * float(ntf[0]) * dose * I0 * np.sqrt(float(dqe[0]))
+    assert abs(out.real.mean() / want - 1) < 2e-3
+    assert np.abs(out.imag).max() < 1e-9 * want
+    assert st[0] == pytest.approx(np.abs(out).max()) and st[1] == pytest.approx(np.abs(out).min())
+    assert st[2] == pytest.approx(np.abs(out).std(), rel=1e-9)
+
+
+def test_detector_flat_curves_give_rounded_counts_with_shot_noise():
+    N, I0, dose = 128, 0.5, 400.0
+    ones = np.ones(725, np.float32)
+    out, _ = O.detector_image(np.full((N, N), I0), dose, 11, ones, ones)
+    c = out.real / 64
+    assert np.abs(c - np.round(c)).max() < 1e-9  # integer counts
+    from scipy.stats import norm
+    q = norm.sf(3)
+    gvar = (1 - 2 * q) - 2 * 3 * norm.pdf(3) + 2 * 9 * q
+    m = dose * I0
+    assert abs(c.mean() / m - 1) < 2e-3
+    assert abs(c.var() / (m * gvar + 1 / 12) - 1) < 0.03
+
+
+def test_detector_transfer_of_a_cosine_follows_the_curves():
+    from cudamultislice_amd.detector import tables
+    dqe, ntf = tables()
+    N, eps, k0 = 256, 0.2, 37
+    x = np.arange(N)
+    I = np.tile(1 + eps * np.cos(2 * np.pi * k0 * x / N), (N, 1))
+    out, _ = O.detector_image(I, 1e7, 5, dqe, ntf)
+    F = np.fft.fft2(out.real)
+    ratio = abs(F[0, k0]) / abs(F[0, 0])
+    i = k0 // 4
+    want = eps / 2 * np.sqrt(float(dqe[i]) / float(dqe[0])) * float(ntf[i]) / float(ntf[0])
+    assert ratio == pytest.approx(want, rel=2e-3)
