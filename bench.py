@@ -368,8 +368,18 @@ def main():
     band = 0.5 if inf["band_tiles"] > 0 else 1.0
     bytes_launch = int(16 * B * npix * band) + (8 * npix if kern == "row" else 0)
     achieved = bytes_launch / (ks[kern]["ms_avg"] * 1e-3) / 1e9
+    # measured HBM traffic: FETCH_SIZE / WRITE_SIZE per wave of the same pass
+    # (rocprofv3, calibrated on a plain copy kernel), scaled to this batch
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_stem512.json")
+    if os.path.exists(tpath):
+        tk = json.load(open(tpath))["kernels"].get(kern)
+        if tk and band < 1.0:
+            traffic = int((tk["fetch_bytes_per_wave"] + tk["write_bytes_per_wave"]) * B)
     roof = {"bound": "hbm", "kernel": f"{kern} pass (512, batch {B})", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": "profiles/traffic_stem512.json (rocprofv3 FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, "
+                              "per wave of a 512-wave launch)" if traffic else None,
             "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": ks[kern]["ms_avg"],
             "other_passes": {k: ks[k]["ms_avg"] for k in ("row", "column", "detector", "potential")}}
 

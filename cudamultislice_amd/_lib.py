@@ -95,6 +95,8 @@ _SIGS = {
     "msl_potential_slice": ([_vp, _i, _f, _i, _vp], _i),
     "msl_set_wave": ([_vp, _i, _vp], _i), "msl_get_wave": ([_vp, _i, _vp], _i),
     "msl_set_detector": ([_vp, _i, C.c_uint64], _i), "msl_get_detector_image": ([_vp, _vp], _i),
+    "msl_write_tiff": ([_cp, _i, _i, _vp], _i), "msl_save_tiff": ([_vp, _i, _cp], _i),
+    "msl_cs_df_sweep": ([_vp, _f, _f, _i, _f, _f, _i, _vp, _cp], _i),
     "msl_readout_noise": ([_vp, C.c_longlong, C.POINTER(_f)], _i),
     "msl_server_create": ([C.POINTER(_vp), _i], _i), "msl_server_destroy": ([_vp], None),
     "msl_server_last_error": ([_vp], _cp),

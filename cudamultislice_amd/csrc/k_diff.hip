@@ -66,7 +66,10 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
     for (int c = 0; c < E; ++c) {
       const bool band = !PRUNE || band_block<E>(c);
       gu[c] = (ok && band) ? a.G[ro + t + c * T] : make_float2(0.f, 0.f);
-      w[c] = (ok && (REAL_Z || band)) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
+      // the stored real-space waves are read once: past the caches, so the
+      // gradient rows stay resident for the next column pass
+      if constexpr (REAL_Z) w[c] = ok ? nt_load(&a.Z[ro + t + c * T]) : make_float2(0.f, 0.f);
+      else w[c] = (ok && band) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
     }
     fft_line<Plan<N>, N, +1, S>(gu, t, a.tw, L);              // G_u = IF_x(G_a)
     if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S>(w, t, a.tw, L);  // w = IF_x(z_{i-1})

@@ -42,8 +42,10 @@ static int lg2(int n) {
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 2 GiB of
-// waves): rows 512 and 1024 take the alternative plan (1.27x and 1.02x)
-static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 2, 1};
+// waves): rows of 512 take the alternative plan (1.27x); at 1024 it is 1.09x
+// faster with a precomputed transmission but holds 256 VGPRs (one wave per
+// SIMD) when the row pass evaluates exp(i sigma V) itself, so 1024 keeps 1
+static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 1, 1};
 static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
@@ -144,10 +146,11 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
   const LL L{lds + ln * LS, 1};
   if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, a.tw, L);
   if constexpr ((FL & STORE_AUX) != 0) {
+    // read back only much later (by the adjoint pass): streamed past the caches
     if (ok) {
       float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
-      for (int c = 0; c < E; ++c) row[t + c * T] = v[c];
+      for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
     }
   }
   if constexpr (MUL == RM_T) {

@@ -623,7 +623,7 @@ __global__ void k_ctf(CtfArgs a) {
   float2 out = make_float2(0.f, 0.f);
   if (k2f < objap2) {
     const double lam = a.wavel, k2 = (double)k0x * k0x + (double)k0y * k0y, k = sqrt(k2);
-    const double df = a.dfs[d], cs = a.Cs, b = a.beta, dl = a.delta, p1 = 3.141593;
+    const double df = a.dfs[d], cs = a.css ? a.css[d] : a.Cs, b = a.beta, dl = a.delta, p1 = 3.141593;
     const double factor = b * b / (4.0 * lam * lam);
     const double esa = p1 * k * lam * 2.0 * df + 2.0 * p1 * lam * lam * lam * cs * k2 * k;
     const double es = exp(-factor * esa * esa);

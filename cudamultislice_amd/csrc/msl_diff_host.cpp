@@ -48,7 +48,6 @@ template <class T> struct Buf {
 
 struct msl_diff {
   int device = 0;
-  hipStream_t own = nullptr;
   std::string err;
   int nx = 0, ny = 0, ns = 0, B = 0, K = 1, S = 1;
   float ps = 0, kV = 0, dz = 0, wavel = 0, sigma = 0, kmax = 0, nf = 0;
@@ -113,7 +112,6 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
   try {
     d->device = device;
     HC(hipSetDevice(device));
-    HC(hipStreamCreateWithFlags(&d->own, hipStreamNonBlocking));
     d->nx = nx; d->ny = ny; d->ps = ps; d->kV = kV; d->dz = dz; d->ns = nslices; d->B = nprobe;
     d->K = std::max(1, std::min(K <= 0 ? nslices : K, nslices));
     d->S = (nslices + d->K - 1) / d->K;
@@ -182,10 +180,7 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
 void msl_diff_destroy(msl_diff* d) {
   if (!d) return;
   (void)hipSetDevice(d->device);
-  if (d->own) {
-    (void)hipStreamSynchronize(d->own);
-    (void)hipStreamDestroy(d->own);
-  }
+  (void)hipDeviceSynchronize();  // work queued on callers' streams may still use the buffers
   delete d;
 }
 
@@ -207,7 +202,9 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
   if (!d || !V || !psi0 || !inten) return MSL_ERR_ARG;
   try {
     HC(hipSetDevice(d->device));
-    hipStream_t s = stream ? (hipStream_t)stream : d->own;
+    // the caller's stream; NULL is the null stream, ordered with the other
+    // default-stream work of the process (as torch's default stream)
+    const hipStream_t s = (hipStream_t)stream;
     d->V = V;
     d->psi0 = (const float2*)psi0;
     // z_K, z_2K, ... go to the checkpoints; the last segment's real-space
@@ -238,7 +235,9 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
   }
   try {
     HC(hipSetDevice(d->device));
-    hipStream_t s = stream ? (hipStream_t)stream : d->own;
+    // the caller's stream; NULL is the null stream, ordered with the other
+    // default-stream work of the process (as torch's default stream)
+    const hipStream_t s = (hipStream_t)stream;
     const long long np = d->np();
     // G at the exit: 2 dL/dI psi_n, then the adjoint of the final IF_x
     HC(launch_seed_adjoint(d->exitw.p, dLdI, d->G.p, np, d->B, s));

@@ -24,6 +24,19 @@ __host__ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * c
 __host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 
 // a * W4 with W4 = exp(DIR * 2 pi i / 4): forward (DIR=-1) is -i, inverse +i
+// streaming (non-temporal) access to complex values that are not reused soon
+typedef float msl_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 nt_load(const float2* p) {
+  const msl_f2v v = __builtin_nontemporal_load(reinterpret_cast<const msl_f2v*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void nt_store(float2 v, float2* p) {
+  msl_f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<msl_f2v*>(p));
+}
+
 template <int DIR> __host__ __device__ __forceinline__ float2 rot4(float2 a) {
   return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }

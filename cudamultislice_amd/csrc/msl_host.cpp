@@ -842,18 +842,22 @@ msl_status msl_get_tds(msl_sim* s, float* out) {
 
 namespace {
 // CTF image(s) of the current wave, U.cu:2322-2343 with imagingKernel2
-void ctf_images(msl_sim* s, int ndf, const float* dfs, float* d_out) {
+void ctf_images(msl_sim* s, int ndf, const float* dfs, float* d_out, const float* css = nullptr) {
   const long long n = (long long)s->npix();
   HIPC(hipMemcpyAsync(s->B.p, s->Psi.p, n * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
   fft2(s, s->B.p, 1, false, s->nf);
   DBuf<float2> K;
-  DBuf<float> dd;
+  DBuf<float> dd, dc;
   K.ensure((size_t)ndf * n);
   dd.ensure(ndf);
   HIPC(hipMemcpyAsync(dd.p, dfs, ndf * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  if (css) {
+    dc.ensure(ndf);
+    HIPC(hipMemcpyAsync(dc.p, css, ndf * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  }
   CtfArgs c{};
   c.in = s->B.p; c.out = K.p; c.nx = s->resX; c.ny = s->resY; c.ndf = ndf; c.kx = s->kx.p; c.ky = s->ky.p;
-  c.dfs = dd.p; c.Cs = s->Cs; c.A1 = s->A1; c.A1phi = s->A1phi; c.A2 = s->A2; c.A2phi = s->A2phi; c.obj = s->obj;
+  c.dfs = dd.p; c.css = css ? dc.p : nullptr; c.Cs = s->Cs; c.A1 = s->A1; c.A1phi = s->A1phi; c.A2 = s->A2; c.A2phi = s->A2phi; c.obj = s->obj;
   c.wavel = s->wavel; c.beta = s->beta / 1000.f; c.delta = s->delta; c.scale = 1.0f;
   HIPC(launch_ctf(c, s->stream));
   fft2(s, K.p, ndf, true, s->nf);
@@ -970,6 +974,41 @@ msl_status msl_ctf_sweep(msl_sim* s, int ndf, const float* dfs, float* out) {
   d.ensure((size_t)ndf * s->npix());
   ctf_images(s, ndf, dfs, d.p);
   d2h(s, out, d.p, (size_t)ndf * s->npix() * sizeof(float));
+  API_END(s)
+}
+
+// the GUI's batch Cs / defocus dialog (BatchProcessing.xaml.cs:29-135, run
+// handler left empty by the reference): a grid of CTF images of the current
+// wave, computed in chunks of at most 64 images
+msl_status msl_cs_df_sweep(msl_sim* s, float cs0, float cs1, int ncs, float df0, float df1, int ndf, float* out,
+                           const char* folder) {
+  API_BEGIN(s)
+  require(s->have_waves, MSL_ERR_STATE, "no wave");
+  require(ncs >= 1 && ndf >= 1 && (out || folder), MSL_ERR_ARG, "bad sweep");
+  auto val = [](float a, float b, int n, int i) { return n > 1 ? a + (b - a) * (float)i / (float)(n - 1) : a; };
+  const long long np = (long long)s->npix();
+  const int total = ncs * ndf, chunk = 64;
+  std::vector<float> cs, df, img((size_t)std::min(total, chunk) * np);
+  for (int i = 0; i < ncs; ++i)
+    for (int j = 0; j < ndf; ++j) {
+      cs.push_back(val(cs0, cs1, ncs, i));
+      df.push_back(val(df0, df1, ndf, j));
+    }
+  DBuf<float> d;
+  d.ensure((size_t)std::min(total, chunk) * np);
+  for (int k0 = 0; k0 < total; k0 += chunk) {
+    const int m = std::min(chunk, total - k0);
+    ctf_images(s, m, df.data() + k0, d.p, cs.data() + k0);
+    d2h(s, img.data(), d.p, (size_t)m * np * sizeof(float));
+    if (out) std::memcpy(out + (size_t)k0 * np, img.data(), (size_t)m * np * sizeof(float));
+    for (int k = 0; folder && k < m; ++k) {
+      char name[96];
+      std::snprintf(name, sizeof(name), "/Cs_%g_df_%g.tiff", cs[k0 + k], df[k0 + k]);
+      const msl_status st = msl_write_tiff((std::string(folder) + name).c_str(), s->resX, s->resY,
+                                           img.data() + (size_t)k * np);
+      require(st == MSL_OK, st, std::string("cannot write into ") + folder);
+    }
+  }
   API_END(s)
 }
 

@@ -157,6 +157,7 @@ struct CtfArgs {
   const float* kx;
   const float* ky;
   const float* dfs;
+  const float* css;  // optional per-image Cs (null: Cs for every image)
   float Cs, A1, A1phi, A2, A2phi, obj, wavel, beta, delta, scale;
 };
 hipError_t launch_ctf(const CtfArgs& a, hipStream_t s);

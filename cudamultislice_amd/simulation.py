@@ -163,6 +163,20 @@ class MultisliceSimulation:
         self._c(lib().msl_readout_noise(self._h, counter, C.byref(v)))
         return v.value
 
+    TIFF_IMAGE, TIFF_EW_RE, TIFF_EW_IM, TIFF_TDS = 0, 1, 2, 3
+
+    def save_tiff(self, what: int, path: str):
+        """One of the GUI's 32-bit float TIFF exports (MW.cs:684-890)."""
+        self._c(lib().msl_save_tiff(self._h, what, path.encode()))
+
+    def cs_df_sweep(self, cs=(0.0, 0.0, 1), df=(0.0, 0.0, 1), folder: str | None = None):
+        """CTF images over a Cs x defocus grid ((start, end, steps) each); with
+        folder, also one 32-bit TIFF per image."""
+        o = np.zeros((cs[2], df[2], self.resY, self.resX), np.float32)
+        self._c(lib().msl_cs_df_sweep(self._h, cs[0], cs[1], cs[2], df[0], df[1], df[2], _p(o),
+                                      folder.encode() if folder else None))
+        return o
+
     def ctf_sweep(self, dfs):
         dfs = np.ascontiguousarray(dfs, np.float32)
         o = np.zeros((dfs.size, self.resY, self.resX), np.float32)
@@ -259,3 +273,13 @@ def fft2_device(ptr: int, nx: int, ny: int, nbatch: int, inverse: bool, stream: 
     st = lib().msl_fft2_device(nx, ny, nbatch, C.c_void_p(ptr), 1 if inverse else 0, C.c_void_p(stream))
     if st != 0:
         raise MslError(st, "msl_fft2_device")
+
+
+def write_tiff(path: str, image) -> None:
+    """32-bit float TIFF of a 2-D array, as the GUI's exports write them."""
+    a = np.ascontiguousarray(image, np.float32)
+    if a.ndim != 2:
+        raise ValueError("a 2-D image is required")
+    st = lib().msl_write_tiff(path.encode(), a.shape[1], a.shape[0], _p(a))
+    if st != 0:
+        raise MslError(st, f"cannot write {path}")

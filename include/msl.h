@@ -138,6 +138,14 @@ msl_status msl_get_detector_image(msl_sim* sim, float* out);
 msl_status msl_readout_noise(const msl_sim* sim, long long counter, float* out);
 /* CTF images for several defoci from the current wave: out is ndf*resX*resY */
 msl_status msl_ctf_sweep(msl_sim* sim, int ndf, const float* dfs, float* out);
+/* the GUI's batch Cs / defocus dialog (BatchProcessing.xaml.cs:29-135, whose
+ * run handler the reference leaves empty): CTF images of the current wave for
+ * every (Cs, df) of cs_steps x df_steps values start + i (end - start) /
+ * (steps - 1) (Cs in A); out (optional) is [cs_steps][df_steps][resY][resX];
+ * with folder != NULL each image is also written as
+ * <folder>/Cs_<Cs>_df_<df>.tiff (32-bit float) */
+msl_status msl_cs_df_sweep(msl_sim* sim, float cs_start, float cs_end, int cs_steps, float df_start, float df_end,
+                           int df_steps, float* out, const char* folder);
 /* annular dark-field signal of the current wave: sum of |FFT_unitary(psi)|^2
  * over inner <= lambda |k| < outer (mrad) */
 msl_status msl_adf(msl_sim* sim, float inner_mrad, float outer_mrad, float* out);
@@ -195,7 +203,9 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
                            int nprobe, int checkpoint_every);
 void msl_diff_destroy(msl_diff* d);
 const char* msl_diff_last_error(const msl_diff* d);
-/* device pointers; stream is a hipStream_t (NULL = the handle's own) */
+/* device pointers; work is queued on `stream` (a hipStream_t; NULL is the
+ * null stream, ordered with all other default-stream work such as torch's
+ * default stream) and not waited for */
 msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float* intensity, void* stream);
 msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream);
 msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, float* kmax, long long* workspace_bytes);
@@ -215,6 +225,18 @@ msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* c
 /* plain streaming device copy (bytes a multiple of 16); the reference point
  * for hardware traffic counters */
 msl_status msl_copy_device(void* dst, const void* src, long long bytes, void* stream);
+
+/* ---- on-disk output (the GUI's TIFF exports, MW.cs:684-890) ----
+ * 32-bit float TIFF: one sample per pixel, IEEE float, one strip, top-left
+ * origin, min-is-black, uncompressed, little-endian.  msl_write_tiff writes a
+ * caller's height x width image (host only); msl_save_tiff writes one of the
+ * GUI's exports of the handle's state: the simulated image as GetImValue
+ * reads it (detector image + readout noise, or the CTF image with the
+ * detector model off), the exit wave's real or imaginary part, or the CBED
+ * accumulator. */
+enum { MSL_TIFF_IMAGE = 0, MSL_TIFF_EW_RE = 1, MSL_TIFF_EW_IM = 2, MSL_TIFF_TDS = 3 };
+msl_status msl_write_tiff(const char* path, int width, int height, const float* data);
+msl_status msl_save_tiff(msl_sim* sim, int what, const char* path);
 
 /* ---- serving: a mix of independent jobs of uneven size (no counterpart in
  * the reference, whose GUI runs one simulation at a time, MW.cs:231-403) ----

@@ -201,6 +201,36 @@ def test_tem_si_1024_252_slices_full_size_properties(fp):
     assert len({round(float(s.std()), 6) for s in sweep}) == len(dfs)  # defocus changes the image
 
 
+def test_tiff_exports_hold_the_handle_state(fp, tmp_path):
+    """The GUI's four TIFF exports (MW.cs:684-890) read back with PIL equal
+    the exit wave, the CBED accumulator and GetImValue's image."""
+    from PIL import Image
+    Z, x, y, z = _si_small()
+    g, _ = _both(Z, x, y, z, fp)
+    g.UploadBinnedAtoms()
+    g.ApplyMicroscopeParameters(200.0, -500.0, 0, 0, 1.2e7, 0, 0, 20.0)
+    g.SetCalculationVariables(0.2, -500.0, 64, 64, 1, 1, 20, 2.0, 2.0)
+    g.PreCalculateFrequencies()
+    g.InitialiseWavefunctions(1.0)
+    g.AllocTDS()
+    for it in range(1, 6):
+        g.MultisliceStepConv(it)
+    ew = g.GetExitWave()
+    g.AddTDSWaves()
+    g.set_detector(True, 9)
+    g.SimulateImage(500.0)
+    read = lambda p: np.asarray(Image.open(p), np.float32)  # noqa: E731
+    for what, want in ((g.TIFF_EW_RE, ew.real), (g.TIFF_EW_IM, ew.imag), (g.TIFF_TDS, g.GetTDS())):
+        p = str(tmp_path / f"{what}.tiff")
+        g.save_tiff(what, p)
+        assert np.array_equal(read(p), np.asarray(want, np.float32)), what
+    p = str(tmp_path / "image.tiff")
+    g.save_tiff(g.TIFF_IMAGE, p)
+    img = read(p)
+    noise = np.array([g.readout_noise(i) for i in range(img.size)], np.float32).reshape(img.shape)
+    assert np.allclose(img - noise, np.abs(g.detector_image()), rtol=1e-6, atol=1e-3)
+
+
 def test_stem_probe_steps_and_tds_match_oracle(fp):
     Z, x, y, z = _si_small()
     g, o = _both(Z, x, y, z, fp)
@@ -228,7 +258,7 @@ def test_stem_probe_steps_and_tds_match_oracle(fp):
     assert rel_l2(g.GetTDS(), o.tds()) < 1e-4
 
 
-def test_ctf_image_matches_oracle(fp):
+def test_ctf_image_matches_oracle(fp, tmp_path):
     Z, x, y, z = _si_small()
     g, o = _both(Z, x, y, z, fp)
     g.UploadBinnedAtoms()
@@ -247,6 +277,17 @@ def test_ctf_image_matches_oracle(fp):
     sw = g.ctf_sweep(dfs)
     for k, d in enumerate(dfs):
         assert rel_l2(sw[k], o.ctf_image(float(d))) < 1e-4
+    # batch Cs x defocus grid, images also written as TIFF files
+    import os
+    from PIL import Image
+    grid = g.cs_df_sweep(cs=(0.0, 2.4e7, 2), df=(-800.0, 200.0, 3), folder=str(tmp_path))
+    for i, csv in enumerate((0.0, 2.4e7)):
+        o.apply_microscope(200.0, -500.0, 20.0, 0.3, csv, 0.5, 30.0, 20.0)
+        for j, d in enumerate((-800.0, -300.0, 200.0)):
+            assert rel_l2(grid[i, j], o.ctf_image(d)) < 1e-4, (csv, d)
+            f = tmp_path / f"Cs_{csv:g}_df_{d:g}.tiff"
+            assert np.array_equal(np.asarray(Image.open(f), np.float32), grid[i, j])
+    assert len(os.listdir(tmp_path)) == 6
 
 
 def test_detector_model_matches_oracle(fp):

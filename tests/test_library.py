@@ -44,3 +44,23 @@ def test_host_only_entry_points():
 def test_embedded_table_is_what_the_oracle_gets(fp):
     assert fp.shape == (103, 12) and fp.dtype == np.float32
     assert np.all(fp[:, 1::2] > 0)  # b_i, d_i widths are positive
+
+
+@needs_lib
+def test_tiff_writer_round_trips_through_an_independent_reader(tmp_path):
+    from PIL import Image
+    from cudamultislice_amd import write_tiff
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal((48, 80)) * 1e3).astype(np.float32)
+    a[0, 0], a[-1, -1] = np.float32(-0.0), np.float32(1e-30)
+    p = str(tmp_path / "x.tiff")
+    write_tiff(p, a)
+    im = Image.open(p)
+    tags = {k: (v[0] if isinstance(v, tuple) else v) for k, v in im.tag_v2.items()}  # before decoding
+    assert tags[339] == 3 and tags[258] == 32 and tags[259] == 1 and tags[274] == 1 and tags[262] == 1
+    assert tags[278] == 48  # one strip
+    assert im.mode == "F" and im.size == (80, 48)
+    assert np.array_equal(np.asarray(im, np.float32), a)
+    L = _lib.lib()
+    assert L.msl_write_tiff(p.encode(), 0, 4, a.ctypes.data_as(C.c_void_p)) == 1
+    assert L.msl_write_tiff(str(tmp_path / "no" / "such.tiff").encode(), 4, 4, a.ctypes.data_as(C.c_void_p)) == 3
