@@ -70,6 +70,7 @@ _SIGS = {
     "msl_stem_initialise_wavefunctions": ([_vp, _i, _i], _i),
     "msl_set_conventional_dz": ([_vp, _f], _i),
     "msl_multislice_step": ([_vp, _i], _i), "msl_multislice_step_conv": ([_vp, _i], _i),
+    "msl_multislice_step_rs": ([_vp, _i], _i),
     "msl_get_slices": ([_vp, _i, C.POINTER(_i)], _i),
     "msl_get_exit_wave": ([_vp, _vp], _i), "msl_get_stem_exit_wave": ([_vp, _vp], _i),
     "msl_alloc_tds": ([_vp], _i), "msl_add_tds_waves": ([_vp], _i), "msl_get_tds": ([_vp, _vp], _i),

@@ -78,7 +78,7 @@ void UnmanagedMultisliceSimulation::AllocTDS() {
 }
 void UnmanagedMultisliceSimulation::MultisliceStep(int it) { Sync(msl_multislice_step(sim_, it)); }
 void UnmanagedMultisliceSimulation::MultisliceStepConv(int it) { Sync(msl_multislice_step_conv(sim_, it)); }
-void UnmanagedMultisliceSimulation::MultisliceStepRS(int) { status_ = MSL_ERR_UNSUPPORTED; }
+void UnmanagedMultisliceSimulation::MultisliceStepRS(int it) { Sync(msl_multislice_step_rs(sim_, it)); }
 void UnmanagedMultisliceSimulation::AllocHostImage() {
   delete[] SimulatedImage;
   SimulatedImage = new msl_float2[(size_t)resX_ * resY_]();

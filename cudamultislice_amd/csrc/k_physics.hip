@@ -244,6 +244,11 @@ __device__ __forceinline__ float sample_z(float zf, int m, float dz) {
 #pragma clang fp contract(off)
   return zf - ((float)m * dz) / 20.0f;
 }
+// binnedAtomPKernelRS samples z - m dz / 10 with an integer dz of 1 (U.cu:700)
+__device__ __forceinline__ float sample_z_rs(float zf, int m) {
+#pragma clang fp contract(off)
+  return zf - (float)m / 10.0f;
+}
 
 constexpr int PT = 16;      // pixel tile edge
 constexpr int ACAP = 1024;  // atoms staged in LDS per chunk
@@ -266,7 +271,7 @@ __global__ void __launch_bounds__(256) k_potential(PotArgs a) {
   {
 #pragma clang fp contract(off)
     h = a.MaxZ - a.MinZ;
-    zf = (a.mode == 2) ? a.zplane : h - a.dz * (float)(a.first_slice + slice);
+    zf = (a.mode >= 2) ? a.zplane : h - a.dz * (float)(a.first_slice + slice);
   }
   int k0, k1;
   if (a.mode == 1) {
@@ -331,6 +336,12 @@ __global__ void __launch_bounds__(256) k_potential(PotArgs a) {
             }
           } else if (a.mode == 1) {
             sum += vzatom_dev(c, rxy);
+          } else if (a.mode == 3) {  // real-space slab of 1 A, 10 samples
+            const float zr = sz[l];
+            for (int m = 1; m <= 10; ++m) {
+              const float r2 = dist2_z(rxy, sample_z_rs(zf, m), zr);
+              if (r2 < 5.0f) sum += 0.1f * vatom_dev(c, r2);
+            }
           } else {
             const float r2 = dist2_z(rxy, zf, sz[l]);
             if (r2 < 5.0f) sum += vatom_dev(c, r2);
@@ -536,6 +547,42 @@ __global__ void k_intensity(const float2* w, float* o, long long n, float f) {
 }
 hipError_t launch_intensity(const float2* w, float* o, long long n, float f, hipStream_t s) {
   hipLaunchKernelGGL(k_intensity, dim3(nblk(n, 256)), dim3(256), 0, s, w, o, n, f);
+  return hipGetLastError();
+}
+
+// ============================================================== real-space multislice
+// One term of MultisliceStepRS's series (RSMS with DifferentialXKernel /
+// DifferentialYKernel, U.cu:798-877): out = w + i pre (Lx w + Ly w + V w)
+// with L (4/3 d1 - d2/12) / ps^2 per axis, d1 = f[+1] + f[-1] - f[0],
+// d2 = f[+2] + f[-2] - f[0], and neighbours outside the grid reading element
+// 0 of the row / column; the first term (w == null) is 1 + i pre V.
+__global__ void k_rs_term(const float2* w, float2* out, const float2* V, int nx, int ny, float ps, float pre) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= nx) return;
+  const long long row = (long long)y * nx, id = row + x;
+  const float v = V[id].x;
+  if (!w) {
+    out[id] = make_float2(1.0f, pre * v);
+    return;
+  }
+  const int xp1 = x > 0 ? x - 1 : 0, xp2 = x > 1 ? x - 2 : 0, xn1 = x < nx - 1 ? x + 1 : 0, xn2 = x < nx - 2 ? x + 2 : 0;
+  const int yp1 = y > 0 ? y - 1 : 0, yp2 = y > 1 ? y - 2 : 0, yn1 = y < ny - 1 ? y + 1 : 0, yn2 = y < ny - 2 ? y + 2 : 0;
+  const float2 f = w[id];
+  const float2 a1 = w[row + xn1], b1 = w[row + xp1], a2 = w[row + xn2], b2 = w[row + xp2];
+  const float2 c1 = w[(long long)yn1 * nx + x], e1 = w[(long long)yp1 * nx + x];
+  const float2 c2 = w[(long long)yn2 * nx + x], e2 = w[(long long)yp2 * nx + x];
+  const float s = 1.0f / (ps * ps);
+  const float dxx = s * (4.0f * ((a1.x + b1.x) - f.x) / 3.0f - ((a2.x + b2.x) - f.x) / 12.0f);
+  const float dxy = s * (4.0f * ((a1.y + b1.y) - f.y) / 3.0f - ((a2.y + b2.y) - f.y) / 12.0f);
+  const float dyx = s * (4.0f * ((c1.x + e1.x) - f.x) / 3.0f - ((c2.x + e2.x) - f.x) / 12.0f);
+  const float dyy = s * (4.0f * ((c1.y + e1.y) - f.y) / 3.0f - ((c2.y + e2.y) - f.y) / 12.0f);
+  const float r1x = -1.0f * pre * (dxy + dyy + v * f.y);
+  const float r1y = pre * (dxx + dyx + v * f.x);
+  out[id] = make_float2(f.x + r1x, f.y + r1y);
+}
+hipError_t launch_rs_term(const float2* w, float2* out, const float2* V, int nx, int ny, float ps, float pre,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_rs_term, dim3((nx + 255) / 256, ny), dim3(256), 0, s, w, out, V, nx, ny, ps, pre);
   return hipGetLastError();
 }
 

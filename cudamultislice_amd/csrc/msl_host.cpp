@@ -775,6 +775,39 @@ msl_status msl_multislice_step(msl_sim* s, int it) {
   API_END(s)
 }
 
+// MultisliceStepRS, U.cu:2126-2178: real potential of the 1 A slab below
+// z = H - it, the 10-term series from 1 (RSMS), then PsiPlus = result;
+// after the first step PsiMinus = PsiPlus * Psi; Psi = PsiPlus.
+msl_status msl_multislice_step_rs(msl_sim* s, int it) {
+  API_BEGIN(s)
+  require(s->have_waves, MSL_ERR_STATE, "InitialiseWavefunctions first");
+  require(it >= 1, MSL_ERR_ARG, "iterations count from 1");
+  require(s->cancel == 0, MSL_ERR_CANCELLED, "cancelled");
+  const long long n = (long long)s->npix();
+  float zc;
+  {
+    const float h = s->MaxZ - s->MinZ;
+    const float d = 1.0f * (float)it;
+    zc = h - d;
+  }
+  potential(s, 3, 0, 1, 1.0f, zc, 1, s->V.p, n);
+  float2 *a = s->PsiPlus.p, *b = s->G.p;
+  for (int m = 1; m <= 10; ++m) {
+    const float pre = (1.0f / (float)m) * (s->wavel * 1.0f / (4.0f * 3.1415927f));
+    if (m == 1) {
+      HIPC(launch_rs_term(nullptr, a, s->V.p, s->resX, s->resY, s->ps, pre, s->stream));
+    } else {
+      HIPC(launch_rs_term(a, b, s->V.p, s->resX, s->resY, s->ps, pre, s->stream));
+      std::swap(a, b);
+    }
+  }
+  if (a != s->PsiPlus.p)
+    HIPC(hipMemcpyAsync(s->PsiPlus.p, a, n * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+  if (it != 1) HIPC(launch_cmul(s->PsiPlus.p, s->Psi.p, s->PsiMinus.p, n, 1.0f, s->stream));
+  HIPC(hipMemcpyAsync(s->Psi.p, s->PsiPlus.p, n * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+  API_END(s)
+}
+
 msl_status msl_get_slices(const msl_sim* s, int conventional, int* out) {
   if (!s || !out) return MSL_ERR_ARG;
   if (conventional) *out = (int)std::ceil((s->MaxZ - s->MinZ) / s->conventionaldz);

@@ -115,10 +115,11 @@ struct PotArgs {
   float ps, offx, offy;
   float dz;                     // slice thickness (modes 0, 1)
   int first_slice, nslices;     // slice i uses z = (MaxZ-MinZ) - dz * i (modes 0, 1)
-  float zplane;                 // mode 2 plane position (single slice)
+  float zplane;                 // modes 2, 3: plane position (single slice)
   int loadslicesz;
   float sigma;
-  int mode, out_kind;
+  int mode, out_kind;           // mode 0 3D-integrated slice, 1 projected, 2 FD plane,
+                                // 3 real-space 1 A slab; out_kind 0 exp(i sigma V), 1 (V, 0), 2 V
   void* out;                    // [nslices][ny][nx]
   long long out_sstride;        // elements between slices
 };
@@ -161,6 +162,10 @@ struct CtfArgs {
   float Cs, A1, A1phi, A2, A2phi, obj, wavel, beta, delta, scale;
 };
 hipError_t launch_ctf(const CtfArgs& a, hipStream_t s);
+
+// one term of the real-space multislice series (MultisliceStepRS, U.cu:2126-2178)
+hipError_t launch_rs_term(const float2* w, float2* out, const float2* V, int nx, int ny, float ps, float pre,
+                          hipStream_t s);
 
 // stochastic detector model (SimulateImage, U.cu:2345-2428)
 hipError_t launch_detector_filter(float2* F, int nx, int ny, const float* tab, int binning, int root, float scale,

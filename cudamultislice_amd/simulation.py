@@ -108,6 +108,9 @@ class MultisliceSimulation:
     def MultisliceStep(self, it: int):
         self._c(lib().msl_multislice_step(self._h, it))
 
+    def MultisliceStepRS(self, it: int):
+        self._c(lib().msl_multislice_step_rs(self._h, it))
+
     def MultisliceStepConv(self, it: int):
         self._c(lib().msl_multislice_step_conv(self._h, it))
 

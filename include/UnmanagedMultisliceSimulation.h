@@ -62,7 +62,7 @@ class UnmanagedMultisliceSimulation {
   void AllocTDS();
   void MultisliceStep(int iteration);
   void MultisliceStepConv(int iteration);
-  void MultisliceStepRS(int iteration);  // real-space step: not provided, records MSL_ERR_UNSUPPORTED
+  void MultisliceStepRS(int iteration);
   void AllocHostImage();
   void SimulateImage(float doseperpix);
   void GetExitWave();

@@ -106,6 +106,14 @@ msl_status msl_set_conventional_dz(msl_sim* sim, float dz);
 msl_status msl_multislice_step(msl_sim* sim, int iteration);
 /* MultisliceStepConv, U.cu:2085-2124 */
 msl_status msl_multislice_step_conv(msl_sim* sim, int iteration);
+/* MultisliceStepRS, U.cu:2126-2178 (in the C++ API, not used by the GUI):
+ * real potential of the 1 A slab below z = H - iteration (10 samples), then
+ * the 10-term real-space series 1 + i p1 V, w += i p_m (L w + V w),
+ * p_m = lambda / (4 pi m), L the reference's 5-point second differences with
+ * out-of-grid neighbours reading element 0.  As in the reference the series
+ * restarts from 1 every step: Psi = PsiPlus = w10, and from the second step
+ * PsiMinus = w10 * (previous Psi). */
+msl_status msl_multislice_step_rs(msl_sim* sim, int iteration);
 /* slice count as ManagedMultislice::GetSlices computes it (MM.cpp:105-111) */
 msl_status msl_get_slices(const msl_sim* sim, int conventional, int* out);
 

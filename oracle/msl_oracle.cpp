@@ -290,6 +290,7 @@ void orc_potential(void* p, int mode, double z, double convdz, double* out) {
     else if (mode == 1) s.potential_proj(R(z), R(convdz), t);
     else if (mode == 2) s.potential_fd(R(z), t);
     else if (mode == 3) s.potential_conv(R(z), R(convdz), t, 3);
+    else if (mode == 5) s.potential_rs(R(z), t);
     else s.potential_proj(R(z), R(convdz), t, 3);
     from_complex(t, out);
     return 0;
@@ -313,6 +314,18 @@ void orc_init_stem(void* p, int px, int py, float dz) {
 }
 void orc_step_conv(void* p, int it, int potmode) { dispatch((Handle*)p, [&](auto& s) { s.step_conv(it, potmode); return 0; }); }
 void orc_step_fd(void* p, int it) { dispatch((Handle*)p, [&](auto& s) { s.step_fd(it); return 0; }); }
+void orc_step_rs(void* p, int it) { dispatch((Handle*)p, [&](auto& s) { s.step_rs(it); return 0; }); }
+// the real-space second-derivative stencil of MultisliceStepRS on a wave
+void orc_rs_laplacian(void* p, const double* in, double* out) {
+  dispatch((Handle*)p, [&](auto& s) {
+    using Cx = typename std::decay_t<decltype(s)>::C;
+    std::vector<Cx> w, o;
+    to_complex(in, s.npix(), w);
+    s.rs_laplacian(w, o);
+    from_complex(o, out);
+    return 0;
+  });
+}
 
 void orc_probe(void* p, int px, int py, double* out) {
   dispatch((Handle*)p, [&](auto& s) {

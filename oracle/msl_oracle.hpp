@@ -635,6 +635,82 @@ template <class Real> struct Sim {
     });
   }
 
+  // binnedAtomPKernelRS, U.cu:656-727: 10-point integration of vatom over the
+  // 1 A below z (samples z - m/10, weight 1/10), slabs floor(H-z) +- 3, the
+  // block offset in xy; real potential (no sigma).
+  void potential_rs(Real z, std::vector<C>& out) const {
+    out.assign(npix(), C(0, 0));
+    int kc = slab_of((float)z);
+    int k0 = std::max(0, kc - loadslicesz), k1 = std::min(nSlab - 1, kc + loadslicesz);
+    parallel_for(resY, [&](int yb, int ye) {
+      for (int y = yb; y < ye; ++y)
+        for (int x = 0; x < resX; ++x) {
+          Real px = Real(offx) + Real(x) * Real(ps), py = Real(offy) + Real(y) * Real(ps);
+          float pxf = pix_f(x, ps, offx), pyf = pix_f(y, ps, offy);
+          Real sumz = 0;
+          for_atoms(px, py, k0, k1, [&](int l) {
+            float rxyf = d2xy_f(pxf, pyf, bx[l], by[l]);
+            if (rxyf >= 5.0f) return;
+            Real ddx = px - Real(bx[l]), ddy = py - Real(by[l]);
+            Real rxy = ddx * ddx + ddy * ddy;
+            for (int m = 1; m <= 10; ++m) {
+              volatile float z2f = (float)z - (float)m / 10.0f;
+              if (!(d2z_f(rxyf, z2f, bz[l]) < 5.0f)) continue;
+              Real ddz = (z - Real(m) / Real(10)) - Real(bz[l]);
+              sumz += vatom<Real>(bZ[l], rxy + ddz * ddz, fparams.data()) / Real(10);
+            }
+          });
+          out[(size_t)y * resX + x] = C(sumz, 0);
+        }
+    });
+  }
+
+  // DifferentialXKernel + DifferentialYKernel (U.cu:798-844): per axis
+  // (4/3 d1 - d2/12) / ps^2 with d1 = f[+1] + f[-1] - f[0] and
+  // d2 = f[+2] + f[-2] - f[0] (f[0] subtracted once, as written there);
+  // neighbours outside the grid read element 0 of the row / column.
+  void rs_laplacian(const std::vector<C>& w, std::vector<C>& out) const {
+    out.assign(npix(), C(0, 0));
+    const Real s = Real(1) / (Real(ps) * Real(ps));
+    auto nb = [](int i, int d, int n) {
+      const int j = i + d;
+      return (j >= 0 && j < n) ? j : 0;
+    };
+    for (int y = 0; y < resY; ++y)
+      for (int x = 0; x < resX; ++x) {
+        const C f = w[(size_t)y * resX + x];
+        auto at = [&](int xx, int yy) { return w[(size_t)yy * resX + xx]; };
+        const C d1x = at(nb(x, 1, resX), y) + at(nb(x, -1, resX), y) - f;
+        const C d2x = at(nb(x, 2, resX), y) + at(nb(x, -2, resX), y) - f;
+        const C d1y = at(x, nb(y, 1, resY)) + at(x, nb(y, -1, resY)) - f;
+        const C d2y = at(x, nb(y, 2, resY)) + at(x, nb(y, -2, resY)) - f;
+        out[(size_t)y * resX + x] = s * (Real(4) / Real(3) * (d1x + d1y) - (d2x + d2y) / Real(12));
+      }
+  }
+
+  // MultisliceStepRS, U.cu:2126-2178 with RSMS (U.cu:846-877): the series
+  // w1 = 1 + i p1 V, w_m = w_{m-1} + i p_m (L w_{m-1} + V w_{m-1}) for m = 2..10,
+  // p_m = (1/m) lambda dz / (4 pi) with dz = 1 A and z = H - it.  The series
+  // restarts from 1 every step; the wave becomes PsiPlus = w10 (step 1 keeps
+  // PsiMinus, later steps set PsiMinus = w10 * Psi before Psi = w10).
+  void step_rs(int it) {
+    const float zf = (MaxZ - MinZ) - 1.0f * (float)it;
+    std::vector<C> V, L;
+    potential_rs(Real(zf), V);
+    std::vector<C> w(npix());
+    const Real base = Real(wavel_f) * Real(1) / (Real(4) * Real(3.1415927f));
+    for (size_t i = 0; i < npix(); ++i) w[i] = C(1, base * V[i].real());
+    for (int m = 2; m <= 10; ++m) {
+      rs_laplacian(w, L);
+      const Real pre = base / Real(m);
+      for (size_t i = 0; i < npix(); ++i) w[i] += C(0, pre) * (L[i] + V[i].real() * w[i]);
+    }
+    if (it != 1)
+      for (size_t i = 0; i < npix(); ++i) PsiMinus[i] = w[i] * Psi[i];
+    Psi = w;
+    PsiPlus = w;
+  }
+
   // InitialiseWavefunctions(dz), U.cu:2003-2038 (conventional path: psi = 1).
   void init_plane(float dz) {
     conventionaldz = dz;

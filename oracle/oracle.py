@@ -48,7 +48,8 @@ def _lib():
         "orc_prop": ([vp, d, _dp], None),
         "orc_init_plane": ([vp, f], None), "orc_init_plane_fd": ([vp], None),
         "orc_init_stem": ([vp, i, i, f], None),
-        "orc_step_conv": ([vp, i, i], None), "orc_step_fd": ([vp, i], None),
+        "orc_step_conv": ([vp, i, i], None), "orc_step_fd": ([vp, i], None), "orc_step_rs": ([vp, i], None),
+        "orc_rs_laplacian": ([vp, _dp, _dp], None),
         "orc_probe": ([vp, i, i, _dp], None),
         "orc_get_wave": ([vp, i, _dp], None), "orc_set_wave": ([vp, i, _dp], None),
         "orc_add_tds": ([vp], None), "orc_get_tds": ([vp, _dp], None),
@@ -236,6 +237,13 @@ class OracleSim:
     def init_stem(self, px, py, dz=1.0): L().orc_init_stem(self._h, px, py, dz)
     def step_conv(self, it, potmode=0): L().orc_step_conv(self._h, it, potmode)
     def step_fd(self, it): L().orc_step_fd(self._h, it)
+    def step_rs(self, it): L().orc_step_rs(self._h, it)
+
+    def rs_laplacian(self, w):
+        w = np.ascontiguousarray(w, np.complex128)
+        o = np.zeros_like(w)
+        L().orc_rs_laplacian(self._h, w.view(np.float64).ravel(), o.view(np.float64).ravel())
+        return o
 
     def probe(self, px, py):
         o = self._wave(); L().orc_probe(self._h, px, py, o)

@@ -347,6 +347,24 @@ def _raster(N, n, step):
     return px.ravel().astype(np.int32), py.ravel().astype(np.int32)
 
 
+def test_rs_steps_match_oracle(fp):
+    """MultisliceStepRS: the real-space series and its wave bookkeeping."""
+    Z, x, y, z = _si_small()
+    g, o = _both(Z, x, y, z, fp)
+    g.UploadBinnedAtoms()
+    o.bin()
+    _setup(g, o, 200.0, 0.2, 64, 20.0, off=(2.0, 2.0))
+    g.InitialiseWavefunctions(1.0)
+    o.init_plane(1.0)
+    for it in range(1, 6):
+        g.MultisliceStepRS(it)
+        o.step_rs(it)
+        assert rel_l2(g.get_wave(0), o.wave(0)) < 1e-5, it
+        if it > 1:
+            assert rel_l2(g.get_wave(1), o.wave(1)) < 1e-5, it
+    assert np.std(np.abs(o.wave(0))) > 1e-3  # the potential acted
+
+
 def test_batched_stem_matches_oracle(fp):
     Z, x, y, z = S.srtio3(3, 3, 4)
     g, o = _both(Z, x, y, z, fp)

@@ -587,3 +587,101 @@ def test_detector_transfer_of_a_cosine_follows_the_curves():
     i = k0 // 4
     want = eps / 2 * np.sqrt(float(dqe[i]) / float(dqe[0])) * float(ntf[i]) / float(ntf[0])
     assert ratio == pytest.approx(want, rel=2e-3)
+
+
+# ---------------------------------------------------------------- real-space multislice
+def _brute_rs(s, Z, x, y, z, zc, N, ps, off):
+    """No bins: binnedAtomPKernelRS restated per pixel over all atoms in the
+    +-3 slab window, 10 samples z - m/10 of weight 1/10, r^2 < 5 in FP32."""
+    e = s.extents()
+    H = float(e[4] - e[5])
+    kc = int(np.floor(np.float32(H) - np.float32(zc)))
+    zid = np.floor((np.float32(e[4]) - z) / np.float32(1)).astype(int)
+    X, Y = (x - np.float32(e[1])).astype(np.float64), (y - np.float32(e[3])).astype(np.float64)
+    Zr = (z - np.float32(e[5])).astype(np.float64)
+    ps, off = float(np.float32(ps)), float(np.float32(off))
+    out = np.zeros((N, N))
+    for iy in range(N):
+        for ix in range(N):
+            px, py = off + ix * ps, off + iy * ps
+            pxf, pyf = _f32_fma(ix, ps, off), _f32_fma(iy, ps, off)
+            for a in range(len(Z)):
+                if abs(zid[a] - kc) > 3:
+                    continue
+                dxf, dyf = np.float32(pxf - np.float32(X[a])), np.float32(pyf - np.float32(Y[a]))
+                rxyf = _f32_fma(dyf, dyf, np.float32(dxf * dxf))
+                for m in range(1, 11):
+                    z2f = np.float32(np.float32(zc) - np.float32(m) / np.float32(10))
+                    ddf = np.float32(z2f - np.float32(Zr[a]))
+                    if not _f32_fma(ddf, ddf, rxyf) < np.float32(5):
+                        continue
+                    r2 = (px - X[a]) ** 2 + (py - Y[a]) ** 2 + (zc - m / 10.0 - Zr[a]) ** 2
+                    out[iy, ix] += O.vatom(int(Z[a]), r2, s._fp) / 10.0
+    return out
+
+
+def test_rs_potential_brute_force(fp):
+    Z, x, y, z = _tiny_specimen()
+    s = _sim_with(Z, x, y, z, fp)
+    s._fp = fp
+    N, ps, off = 12, 0.55, 32.0
+    s.apply_microscope(200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+    H = float(s.extents()[4] - s.extents()[5])
+    for zc in (H - 6.0, H - 7.5):
+        zc = float(np.float32(zc))  # the step passes the FP32 plane H - it
+        got = s.potential(5, zc, 1.0).real
+        want = _brute_rs(s, Z, x, y, z, zc, N, ps, off)
+        assert np.max(want) > 1.0
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_rs_stencil_delta_responses_and_boundary_rule(fp):
+    Z, x, y, z = _tiny_specimen()
+    s = _sim_with(Z, x, y, z, fp)
+    N, ps = 16, 0.5
+    s.apply_microscope(200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1, 0.0, 0.0)
+    h2 = 1.0 / (ps * ps)
+    d = np.zeros((N, N), complex)
+    d[7, 6] = 1.0
+    L = s.rs_laplacian(d)
+    # interior: centre 2 (-4/3 + 1/12), first neighbours 4/3, second -1/12 (f subtracted once)
+    assert L[7, 6] == pytest.approx(2 * (-4 / 3 + 1 / 12) * h2)
+    for dy, dx in ((0, 1), (0, -1), (1, 0), (-1, 0)):
+        assert L[7 + dy, 6 + dx] == pytest.approx(4 / 3 * h2)
+        assert L[7 + 2 * dy, 6 + 2 * dx] == pytest.approx(-1 / 12 * h2)
+    assert np.count_nonzero(np.abs(L) > 1e-12) == 9
+    # element 0 of a row stands in for every missing neighbour
+    d = np.zeros((N, N), complex)
+    d[5, 0] = 1.0
+    L = s.rs_laplacian(d)
+    row = L[5]
+    assert row[0] == pytest.approx(2 * (-1 / 12) * h2 * 0 + (-4 / 3 + 1 / 12) * h2 * 0 + (-4 / 3 + 1 / 12) * h2)
+    assert row[1] == pytest.approx((4 / 3 - 1 / 12) * h2)      # x-1 and x-2 both read element 0
+    assert row[2] == pytest.approx(-1 / 12 * h2)
+    assert row[N - 1] == pytest.approx((4 / 3 - 1 / 12) * h2)  # x+1 and x+2 read element 0
+    assert row[N - 2] == pytest.approx(-1 / 12 * h2)
+
+
+def test_rs_step_in_vacuum_follows_the_series_closed_form(fp):
+    # atoms far outside the grid: V = 0, a constant wave sees L w = (5/2) w / ps^2
+    Z = np.array([6, 6], np.int32)
+    s = _sim_with(Z, np.array([-60, 200], np.float32), np.array([-60, 200], np.float32),
+                  np.array([0, 30], np.float32), fp)
+    N, ps = 16, 0.3
+    s.apply_microscope(200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1, 0.0, 0.0)
+    s.init_plane(1.0)
+    lam = float(np.float32(s.scalars()["wavel"]))
+    base = lam / (4 * float(np.float32(3.1415927)))
+    p32 = float(np.float32(ps))  # PixelScale is a float
+    c = 2.5 / (p32 * p32)
+    w = 1.0 + 0j
+    for m in range(2, 11):
+        w *= 1 + 1j * base / m * c
+    for it in (1, 2):
+        s.step_rs(it)
+        assert np.allclose(s.wave(0), w, rtol=1e-12, atol=0)
+        assert np.allclose(s.wave(2), w, rtol=1e-12, atol=0)
+    assert np.allclose(s.wave(1), w * w, rtol=1e-12)  # step 2: PsiMinus = PsiPlus * Psi
