@@ -45,7 +45,7 @@ class msl_job(C.Structure):
 class msl_serve_params(C.Structure):
     _fields_ = [("kV", C.c_float), ("ps", C.c_float), ("dz", C.c_float), ("aperture", C.c_float),
                 ("adf_inner", C.c_float), ("adf_outer", C.c_float), ("potential_mode", C.c_int),
-                ("budget_bytes", C.c_longlong)]
+                ("budget_bytes", C.c_longlong), ("group_sizes", C.c_int)]
 
 
 _vp, _i, _f, _cp = C.c_void_p, C.c_int, C.c_float, C.c_char_p

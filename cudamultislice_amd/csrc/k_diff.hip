@@ -43,8 +43,11 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
   constexpr int T = Plan<N>::T, E = N / T, LINES = Cfg::L, LS = Cfg::LS;
   static_assert(LS * 2 >= N, "the reduction reuses the line buffers");
   __shared__ float2 lds[LINES * LS];
+  __shared__ float2 stw[N];  // twiddles staged in LDS
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   const int y = blockIdx.x, g = blockIdx.y;
+  for (int i = tid; i < N; i += LINES * T) stw[i] = a.tw[i];
+  __syncthreads();
   const LL L{lds + ln * LS, 1};
   const float* vr = a.V + (long long)y * N;
   float2 tr[E];
@@ -71,15 +74,15 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
       if constexpr (REAL_Z) w[c] = ok ? nt_load(&a.Z[ro + t + c * T]) : make_float2(0.f, 0.f);
       else w[c] = (ok && band) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
     }
-    fft_line<Plan<N>, N, +1, S>(gu, t, a.tw, L);              // G_u = IF_x(G_a)
-    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S>(w, t, a.tw, L);  // w = IF_x(z_{i-1})
+    fft_line<Plan<N>, N, +1, S>(gu, t, stw, L);              // G_u = IF_x(G_a)
+    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S>(w, t, stw, L);  // w = IF_x(z_{i-1})
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const float2 gw = cmulc(gu[c], tr[c]);     // conj(t) G_u
       acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
       gu[c] = gw;
     }
-    fft_line<Plan<N>, N, -1, S>(gu, t, a.tw, L);              // G_{z_{i-1}} = F_x(G_w)
+    fft_line<Plan<N>, N, -1, S>(gu, t, stw, L);              // G_{z_{i-1}} = F_x(G_w)
     if (ok) {
 #pragma unroll
       for (int c = 0; c < E; ++c)

@@ -12,6 +12,7 @@
 // chosen by msl_autotune_passes() from timed runs, or by the defaults below.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -39,21 +40,26 @@ static int lg2(int n) {
 //           per CU to overlap the barriers)
 //         4 as 0 with two LDS copies used by alternate exchanges (one
 //           barrier per exchange instead of two)
+//         5 as 0 in persistent workgroups that stream the next tile into a
+//           second LDS buffer (direct global-to-LDS loads) while the current
+//           one is transformed, so the strided column loads overlap compute
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
-// defaults from msl_autotune_passes on MI355X (band-pruned passes, 2 GiB of
-// waves): rows of 512 take the alternative plan (1.27x); at 1024 it is 1.09x
-// faster with a precomputed transmission but holds 256 VGPRs (one wave per
-// SIMD) when the row pass evaluates exp(i sigma V) itself, so 1024 keeps 1
+// defaults from msl_autotune_passes on MI355X (band-pruned passes, 1-4 GiB
+// of waves): rows of 512 take the alternative plan (1.04x over 1); at 1024 the
+// alternative plan holds 256 VGPRs (one wave per SIMD) when the row pass
+// evaluates exp(i sigma V) itself, so 1024 keeps 1.  Columns: 256 the
+// alternative plan (1.07x over 0), 512 half-width tiles (1.05x), 1024 two
+// LDS copies (1.03x), 2048 variant 0.
 static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 1, 1};
-static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 2, 3, 4, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
 void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 3 : 5; }
+int num_pass_variants(int kind) { return kind == 0 ? 3 : 6; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
@@ -97,6 +103,18 @@ int band_tiles_lo(int nx) {
   return nx / 4 / C;
 }
 
+// Workgroups are dispatched round-robin over the XCDs, each with its own L2.
+// A column tile narrower than 16 columns reads part of each 128-byte line;
+// the neighbouring tile reads the rest.  Remapping the linear workgroup id so
+// that every XCD walks a contiguous range of logical ids puts neighbouring
+// tiles in one L2 at about the same time, so each line is fetched once.
+constexpr int kXcds = 8;
+__device__ __forceinline__ long long xcd_remap(long long id, long long total) {
+  const long long whole = total - total % kXcds;
+  if (id >= whole) return id;
+  return (id % kXcds) * (whole / kXcds) + id / kXcds;
+}
+
 // ------------------------------------------------------------------ rows
 template <int N, int RV> struct RowCfg {
   using P = typename std::conditional<RV == 2, AltPlan<N>, Plan<N>>::type;
@@ -121,16 +139,17 @@ template <int E> struct BandBlocks {
   static __host__ __device__ constexpr bool active(int c) { return c < LO || c >= E - LO; }
 };
 
+// the LINES lines of workgroup-sized block blk of a row pass; lds holds
+// LINES * LS elements, tw the twiddles of length N
 template <int N, int MODE, int MUL, int FL, int RV>
-__global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_row(RowArgs a) {
+__device__ __forceinline__ void row_block(const RowArgs& a, long long blk, float2* lds, const float2* stw) {
   using Cfg = RowCfg<N, RV>;
   using P = typename Cfg::P;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
   using LL = LdsLine<Cfg::SH>;
   constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
-  __shared__ float2 lds[LINES * LS];
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
-  const long long line = (long long)blockIdx.x * LINES + ln;
+  const long long line = blk * LINES + ln;
   const bool ok = line < (long long)a.nbatch * a.ny;
   const int b = ok ? (int)(line / a.ny) : 0;
   const int y = ok ? (int)(line - (long long)b * a.ny) : 0;
@@ -144,7 +163,7 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
     }
   }
   const LL L{lds + ln * LS, 1};
-  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, a.tw, L);
+  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, stw, L);
   if constexpr ((FL & STORE_AUX) != 0) {
     // read back only much later (by the adjoint pass): streamed past the caches
     if (ok) {
@@ -176,7 +195,102 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
       for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
     }
   }
-  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, a.tw, L);
+  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
+  if (ok) {
+    float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c)
+      if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];
+  }
+}
+
+// Logical block (wave b, row block yb) = b ybs + yb of hardware workgroup h.
+// Each XCD takes a contiguous range of logical blocks (xcd_remap), and inside
+// it the same row block of kRowWaves consecutive waves runs back to back:
+// those waves read the same transmission rows, which then come from the L2
+// once instead of once per wave (a 2048^2 slice, 32 MiB, does not stay in an
+// XCD's L2 between one wave and the next).
+constexpr int kRowWaves = 8;
+__device__ __forceinline__ long long row_order(long long h, long long total, int nbatch, int ybs) {
+  const long long r = xcd_remap(h, total);
+  const long long span = (long long)kRowWaves * ybs;
+  const long long full = (long long)(nbatch / kRowWaves) * span;  // blocks of complete wave groups
+  if (r >= full) return r;
+  const long long grp = r / span, rem = r - grp * span;
+  const int yb = (int)(rem / kRowWaves), bi = (int)(rem - (long long)yb * kRowWaves);
+  return (grp * kRowWaves + bi) * ybs + yb;
+}
+
+// The kernel keeps its own copy of the row_block body, on its own LDS arrays:
+// reached through pointer arguments the same body is scheduled differently
+// (fewer registers, less overlap) and ran 11 % slower at 512 on MI355X.
+template <int N, int MODE, int MUL, int FL, int RV>
+__global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_row(RowArgs a) {
+  __shared__ float2 lds[RowCfg<N, RV>::LINES * RowCfg<N, RV>::LS];
+  __shared__ float2 stw[N];  // twiddles staged in LDS: no global loads inside the transforms
+  for (int i = threadIdx.x; i < N; i += RowCfg<N, RV>::LINES * RowCfg<N, RV>::T) stw[i] = a.tw[i];
+  __syncthreads();
+  long long blk = blockIdx.x;
+  // wave-interleaved, XCD-aware order where a shared slice outgrows the L2
+  // (measured slower than the plain order at 1024 and below, where the slice
+  // stays in the L2 anyway)
+  if constexpr (N >= 2048 && MUL != RM_NONE)
+    if (N % RowCfg<N, RV>::LINES == 0 && a.ny == N)
+      blk = row_order(blockIdx.x, gridDim.x, a.nbatch, N / RowCfg<N, RV>::LINES);
+
+  using Cfg = RowCfg<N, RV>;
+  using P = typename Cfg::P;
+  using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
+  using LL = LdsLine<Cfg::SH>;
+  constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
+  const int tid = threadIdx.x, t = tid % T, ln = tid / T;
+  const long long line = blk * LINES + ln;
+  const bool ok = line < (long long)a.nbatch * a.ny;
+  const int b = ok ? (int)(line / a.ny) : 0;
+  const int y = ok ? (int)(line - (long long)b * a.ny) : 0;
+  float2 v[E];
+  {
+    const float2* row = a.src + (long long)b * a.src_bstride + (long long)y * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const bool load = ok && (!(FL & PRUNE_IN) || BandBlocks<E>::active(c));
+      v[c] = load ? row[t + c * T] : make_float2(0.f, 0.f);
+    }
+  }
+  const LL L{lds + ln * LS, 1};
+  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, stw, L);
+  if constexpr ((FL & STORE_AUX) != 0) {
+    // read back only much later (by the adjoint pass): streamed past the caches
+    if (ok) {
+      float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
+#pragma unroll
+      for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
+    }
+  }
+  if constexpr (MUL == RM_T) {
+    if (ok) {
+      const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
+      const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
+#pragma unroll
+      for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
+    }
+  } else if constexpr (MUL == RM_PHASE) {
+    if (ok) {
+      const float* vr = a.V + (long long)(b / a.t_group) * a.t_bstride + (long long)y * N;
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        float sn, cs;
+        sincosf(a.sigma * vr[t + c * T], &sn, &cs);
+        v[c] = cscale(cmul(v[c], make_float2(cs, sn)), a.scale);
+      }
+    }
+  } else {
+    if (a.scale != 1.0f) {
+#pragma unroll
+      for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
+    }
+  }
+  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
   if (ok) {
     float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
@@ -186,17 +300,57 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
 }
 
 // ------------------------------------------------------------------ columns
+// CreatePropsKernel, U.cu:419: the propagator is non-zero when
+// sqrt(kx^2 + ky^2) < kmax
+__device__ __forceinline__ bool in_prop_band(float kxv, float kyv, float kmax) {
+  return sqrtf(kxv * kxv + kyv * kyv) < kmax;
+}
+
+// Per column x the rows inside the propagator band are r <= rows[x].x and
+// r >= rows[x].y: |ky| grows from row 0 up to N/2 and from row N-1 down to
+// N/2+1, and the FP32 test above is monotone in |ky|.  bad[x] != 0 flags a
+// column where that shape does not hold.
+__global__ void k_prop_rows(const float* kx, const float* ky, int nx, int ny, float kmax, int2* rows, int* bad) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nx) return;
+  const float kxv = kx[x];
+  int hi = -1;
+  while (hi + 1 < ny && in_prop_band(kxv, ky[hi + 1], kmax)) ++hi;
+  int lo = ny;
+  while (lo - 1 > hi && in_prop_band(kxv, ky[lo - 1], kmax)) --lo;
+  int stray = 0;
+  for (int r = hi + 1; r < lo; ++r) stray |= in_prop_band(kxv, ky[r], kmax) ? 1 : 0;
+  rows[x] = make_int2(hi, lo);
+  bad[x] = stray;
+}
+
+hipError_t make_prop_rows(const float* kx, const float* ky, int nx, int ny, float kmax, int2* rows, hipStream_t s) {
+  if (nx <= 0 || ny <= 0 || !kx || !ky || !rows) return hipErrorInvalidValue;
+  int* bad = nullptr;
+  hipError_t e = hipMalloc(&bad, nx * sizeof(int));
+  if (e != hipSuccess) return e;
+  std::vector<int> h(nx, 1);
+  hipLaunchKernelGGL(k_prop_rows, dim3((nx + 63) / 64), dim3(64), 0, s, kx, ky, nx, ny, kmax, rows, bad);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), bad, nx * sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(bad);
+  if (e != hipSuccess) return e;
+  for (int x = 0; x < nx; ++x)
+    if (h[x]) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
 template <int MUL>
-__device__ __forceinline__ float2 kmul(const ColArgs& a, int x, int r, float kxv, float kyv) {
+__device__ __forceinline__ float2 kmul(const ColArgs& a, int x, int r, float kxv, float kyv, int2 pr) {
   if constexpr (MUL == CM_SCALE) {
     return make_float2(a.scale, 0.f);
   } else if constexpr (MUL == CM_PROP || MUL == CM_PROPCONJ) {
-    // CreatePropsKernel, U.cu:419: inside when sqrt(kx^2 + ky^2) < kmax
-    const float k = sqrtf(kxv * kxv + kyv * kyv);
-    if (!(k < a.kmax)) return make_float2(0.f, 0.f);
+    // inside |k| < kmax: rows tabulated per column by k_prop_rows (no
+    // frequency loads or square roots here, and no branch around the load)
     float2 p = cmul(a.px[x], a.py[r]);
     if (MUL == CM_PROPCONJ) p.y = -p.y;
-    return cscale(p, a.scale);
+    return (r <= pr.x || r >= pr.y) ? cscale(p, a.scale) : make_float2(0.f, 0.f);
   } else if constexpr (MUL == CM_BAND) {
     // BandLimitKernel, U.cu:1037-1040: keeps |k| <= kmax
     const float k = sqrtf(kxv * kxv + kyv * kyv);
@@ -214,42 +368,44 @@ template <int MODE> struct ColModeTraits {
 
 // FFT / multiply / IFFT of one column held canonically by T threads
 template <class P, int N, int MODE, int MUL, class S, class LL>
-__device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, const ColArgs& a, const LL& L) {
+__device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, const ColArgs& a, const LL& L,
+                                         const float2* tw) {
   using Tr = ColModeTraits<MODE>;
   constexpr int T = P::T, E = N / T;
-  if constexpr (Tr::FWD) fft_line<P, N, -1, S>(v, t, a.tw, L);
+  if constexpr (Tr::FWD) fft_line<P, N, -1, S>(v, t, tw, L);
   // frequencies are read only by modes that need them (scale-only passes may
   // run without k-grids)
-  constexpr bool NEEDK = (Tr::MULT && MUL != CM_SCALE);
+  constexpr bool NEEDK = (Tr::MULT && (MUL == CM_BAND || MUL == CM_GRAD));
+  constexpr bool NEEDROWS = (Tr::MULT && (MUL == CM_PROP || MUL == CM_PROPCONJ));
   float kxv = 0.f;
   if constexpr (NEEDK) kxv = a.kx[x];
+  int2 pr = make_int2(0, 0);
+  if constexpr (NEEDROWS) pr = a.prop_rows[x];
   if constexpr (Tr::MULT) {
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int r = t + c * T;
       float kyv = 0.f;
       if constexpr (NEEDK) kyv = a.ky[r];
-      v[c] = cmul(v[c], kmul<MUL>(a, x, r, kxv, kyv));
+      v[c] = cmul(v[c], kmul<MUL>(a, x, r, kxv, kyv, pr));
     }
   } else if (a.scale != 1.0f) {
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
   }
-  if constexpr (Tr::INV) fft_line<P, N, +1, S>(v, t, a.tw, L);
+  if constexpr (Tr::INV) fft_line<P, N, +1, S>(v, t, tw, L);
 }
 
 // variants 0, 1: the C columns of a tile spread over the workgroup, thread
 // (col, t) holds rows t + c T of column col; workgroup barriers
-template <int N, int MODE, int MUL, int CV>
-__global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_WAVES)) k_col(ColArgs a) {
-  using Cfg = ColCfg<N, CV>;
+// launched tile bx (of nbx per wave) of wave b; lds holds Cfg::LDS elements,
+// stw the twiddles of length N
+template <int N, int MODE, int MUL, class Cfg>
+__device__ __forceinline__ void col_block(const ColArgs& a, int bx, int b, int nbx, float2* lds, const float2* stw) {
   constexpr int T = Cfg::T, E = N / T, C = Cfg::C;
-  __shared__ float2 lds[Cfg::LDS];
   const int col = threadIdx.x % C, t = threadIdx.x / C;
   // band-limited runs launch only the tiles at both ends of kx
-  const int tile = (a.tiles_lo > 0 && (int)blockIdx.x >= a.tiles_lo) ? (int)blockIdx.x + a.nx / C - 2 * a.tiles_lo
-                                                                      : (int)blockIdx.x;
-  const int b = blockIdx.y;
+  const int tile = (a.tiles_lo > 0 && bx >= a.tiles_lo) ? bx + a.nx / C - 2 * a.tiles_lo : bx;
   const int x = tile * C + col;
   const long long nx = a.nx;
   float2 v[E];
@@ -262,9 +418,9 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_
   const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
   if constexpr (Cfg::DUAL) {
     const DualLine<LL> D{L, LL{lds + N * C + col, C}};
-    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, D);
+    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, D, stw);
   } else {
-    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, L);
+    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, L, stw);
   }
   if constexpr (MODE == COL_FWD_DETECT) {
     const float kxv = a.kx[x];
@@ -287,7 +443,7 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_
       if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) a.partial[(long long)b * gridDim.x + blockIdx.x] = red[0] * a.det_scale;
+    if (threadIdx.x == 0) a.partial[(long long)b * nbx + bx] = red[0] * a.det_scale;
     if (a.dst == nullptr) return;
   }
   {
@@ -295,6 +451,288 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_
 #pragma unroll
     for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
   }
+}
+
+template <int N, int MODE, int MUL, int CV>
+__global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_WAVES)) k_col(ColArgs a) {
+  using Cfg = ColCfg<N, CV>;
+  __shared__ float2 lds[Cfg::LDS];
+  __shared__ float2 stw[N];  // twiddles staged in LDS
+  for (int i = threadIdx.x; i < N; i += Cfg::THREADS) stw[i] = a.tw[i];
+  __syncthreads();
+  // logical (tile, wave) of this workgroup, XCD-aware
+  const long long gid = xcd_remap((long long)blockIdx.y * gridDim.x + blockIdx.x, (long long)gridDim.x * gridDim.y);
+  col_block<N, MODE, MUL, Cfg>(a, (int)(gid % gridDim.x), (int)(gid / gridDim.x), (int)gridDim.x, lds, stw);
+}
+
+// ------------------------------------------------------------------ streamed columns
+// Variant 5: a persistent workgroup walks the (wave, tile) items of the
+// launch; the [row][col] tile of the next item is fetched straight into the
+// other of two LDS buffers (global_load_lds, 16 bytes per lane: lane l of an
+// instruction lands at base + 16 l, i.e. C / 2 lanes per 16-column row) while
+// the current tile is transformed in registers, the current buffer serving as
+// the exchange space.  Arithmetic is that of variant 0.
+template <int N> struct ColStream {
+  // two tiles of C columns must fit the LDS, a workgroup at most 512 threads
+  // (room for the second buffer's addresses) and at least one wavefront
+  static constexpr int T = Plan<N>::T;
+  static constexpr int C0 = (N >= 1024) ? ColTile0<N>::C : ColTile0<N>::C / 2;
+  static constexpr int C = C0 * T >= 64 ? C0 : 64 / T;
+  static constexpr int THREADS = C * T, E = N / T;
+  static constexpr int LANES_PER_ROW = C / 2;           // 16-byte lanes per tile row
+  static constexpr int ROWS_PER_OP = 64 / LANES_PER_ROW;  // rows one wave instruction covers
+  static constexpr int OPS = N / ROWS_PER_OP;             // instructions per tile
+  static constexpr int WAVES = THREADS / 64;
+  static constexpr int OPS_PER_WAVE = OPS / WAVES;        // = E / 2
+  static_assert(C % 2 == 0 && 64 % LANES_PER_ROW == 0 && OPS % WAVES == 0, "tile does not map onto waves");
+};
+
+typedef __attribute__((address_space(3))) void* lds_vptr;
+
+// s_waitcnt vmcnt(n) leaving the other counters free (GFX9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+template <int n> __device__ __forceinline__ void wait_vmcnt() {
+  static_assert(n >= 0 && n < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations, not for its outstanding global loads (the streamed next tile),
+// which a fenced __syncthreads() would drain.
+struct LdsBarrier {
+  __device__ static __forceinline__ void sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+};
+
+template <int N>
+__device__ __forceinline__ void stream_tile(const ColArgs& a, int item, int ntl, float2* buf) {
+  using S = ColStream<N>;
+  const int b = item / ntl, ti = item - b * ntl;
+  const int tile = (a.tiles_lo > 0 && ti >= a.tiles_lo) ? ti + a.nx / S::C - 2 * a.tiles_lo : ti;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rr = lane / S::LANES_PER_ROW, cc = 2 * (lane % S::LANES_PER_ROW);
+  const float2* src = a.src + (long long)b * a.src_bstride + tile * S::C + cc;
+#pragma unroll
+  for (int k = 0; k < S::OPS_PER_WAVE; ++k) {
+    const int r0 = (wave * S::OPS_PER_WAVE + k) * S::ROWS_PER_OP;
+    __builtin_amdgcn_global_load_lds((const void*)(src + (long long)(r0 + rr) * a.nx),
+                                     (lds_vptr)(buf + r0 * S::C), 16, 0, 0);
+  }
+}
+
+// one item: wait for its tile in cur, transform it, store it; the tile of
+// the following item streams into nxt meanwhile.  cur and nxt are distinct
+// (restrict), which lets the compiler keep the transform's LDS traffic from
+// waiting on the stream into nxt.
+template <int N, int MODE, int MUL>
+__device__ __forceinline__ void stream_item(const ColArgs& a, int it, int next, int items, int ntl,
+                                            float2* __restrict__ cur, float2* __restrict__ nxt, const float2* tw) {
+  using S = ColStream<N>;
+  constexpr int T = S::T, E = S::E, C = S::C;
+  const int col = threadIdx.x % C, t = threadIdx.x / C;
+  if (next < items) {
+    stream_tile<N>(a, next, ntl, nxt);
+    wait_vmcnt<S::OPS_PER_WAVE>();  // this wave's loads of the current tile are done
+  } else {
+    wait_vmcnt<0>();
+  }
+  LdsBarrier::sync();  // every wave's share of the current tile has landed
+  float2 v[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) v[c] = cur[(t + c * T) * C + col];
+  LdsBarrier::sync();  // the exchanges below reuse the current buffer
+  const int b = it / ntl, ti = it - b * ntl;
+  const int tile = (a.tiles_lo > 0 && ti >= a.tiles_lo) ? ti + a.nx / C - 2 * a.tiles_lo : ti;
+  const int x = tile * C + col;
+  col_body<Plan<N>, N, MODE, MUL, LdsBarrier>(v, t, x, a, LdsLine<0>{cur + col, C}, tw);
+  float2* dst = a.dst + (long long)b * a.dst_bstride + x;
+  const long long nx = a.nx;
+#pragma unroll
+  for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
+  LdsBarrier::sync();  // the current buffer is refilled two items later
+}
+
+template <int N, int MODE, int MUL>
+__global__ void __launch_bounds__((ColStream<N>::THREADS)) k_col_stream(ColArgs a, int ntl, int items) {
+  __shared__ float2 buf0[N * ColStream<N>::C];
+  __shared__ float2 buf1[N * ColStream<N>::C];
+  __shared__ float2 stw[N];
+  // trip k takes item k G + r: within a trip each XCD holds a contiguous
+  // range of items (neighbouring tiles share an L2)
+  const int G = gridDim.x;
+  int item = (int)xcd_remap(blockIdx.x, G);
+  if (item >= items) return;
+  for (int i = threadIdx.x; i < N; i += ColStream<N>::THREADS) stw[i] = a.tw[i];
+  __syncthreads();
+  stream_tile<N>(a, item, ntl, buf0);
+  // two items per trip so each buffer plays a fixed role in each half
+  for (; item < items; item += 2 * G) {
+    stream_item<N, MODE, MUL>(a, item, item + G, items, ntl, buf0, buf1, stw);
+    const int it = item + G;
+    if (it >= items) break;
+    stream_item<N, MODE, MUL>(a, it, it + G, items, ntl, buf1, buf0, stw);
+  }
+}
+
+// ------------------------------------------------------------------ grouped (uneven sizes)
+// One launch covers the waves of several grid sizes, one segment per size
+// (256 .. 2048): the workgroups of all segments are concatenated and each runs
+// the body of its block's length.  Rows use 256 threads per workgroup at every
+// length, columns 512; per length the plan and tile are those the tuned
+// per-size passes use where the shape allows.
+template <int N> struct GroupRow { static constexpr int RV = N == 512 ? 2 : 1; };
+template <int N> struct GroupCol {
+  using P = typename std::conditional<N == 256, AltPlan<N>, Plan<N>>::type;
+  static constexpr bool INTERLEAVED = true, DUAL = false;
+  static constexpr int T = P::T, THREADS = 512, C = THREADS / T, SH = 0, LSC = 0, LDS = N * C;
+  static_assert(C * T == THREADS && C <= N / 4, "tile does not fit the band");
+};
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+template <int N> constexpr int group_row_lds() { return RowCfg<N, GroupRow<N>::RV>::LINES * RowCfg<N, GroupRow<N>::RV>::LS; }
+constexpr int kGroupRowLds = cmax(cmax(group_row_lds<256>(), group_row_lds<512>()),
+                                  cmax(group_row_lds<1024>(), group_row_lds<2048>()));
+constexpr int kGroupColLds = cmax(cmax(GroupCol<256>::LDS, GroupCol<512>::LDS), cmax(GroupCol<1024>::LDS, GroupCol<2048>::LDS));
+constexpr int kGroupTw = 2048;
+
+// Segment of workgroup `it` and its twiddles staged in LDS.  The table lives
+// in device memory; its fields are read with a uniform index.  (A persistent
+// loop over the blocks of all segments was measured by the compiler at 256
+// VGPRs with the four bodies in one loop: one workgroup per block keeps the
+// register count of the largest body.)
+template <class A>
+__device__ __forceinline__ int group_seg(const PassGroup<A>* g, long long it, float2* stw, int nthreads) {
+  const int nseg = g->nseg;
+  int s = 0;
+  while (s + 1 < nseg && it >= g->blk0[s + 1]) ++s;
+  const float2* tw = g->a[s].tw;
+  const int n = g->a[s].nx;
+  for (int i = threadIdx.x; i < n; i += nthreads) stw[i] = tw[i];
+  __syncthreads();
+  return s;
+}
+
+template <int MODE, int MUL, int FL>
+__global__ void __launch_bounds__(256) k_row_group(const PassGroup<RowArgs>* g) {
+  __shared__ float2 lds[kGroupRowLds];
+  __shared__ float2 stw[kGroupTw];
+  const long long it = blockIdx.x;
+  if (it >= g->blk0[g->nseg]) return;
+  const int s = group_seg(g, it, stw, 256);
+  const RowArgs& a = g->a[s];
+  const long long blk = it - g->blk0[s];
+  switch (a.nx) {
+    case 256: row_block<256, MODE, MUL, FL, GroupRow<256>::RV>(a, blk, lds, stw); break;
+    case 512: row_block<512, MODE, MUL, FL, GroupRow<512>::RV>(a, blk, lds, stw); break;
+    case 1024: row_block<1024, MODE, MUL, FL, GroupRow<1024>::RV>(a, blk, lds, stw); break;
+    default: row_block<2048, MODE, MUL, FL, GroupRow<2048>::RV>(a, blk, lds, stw); break;
+  }
+}
+
+template <int MODE, int MUL>
+__global__ void __launch_bounds__(512) k_col_group(const PassGroup<ColArgs>* g) {
+  __shared__ float2 lds[kGroupColLds];
+  __shared__ float2 stw[kGroupTw];
+  // XCD-aware: each XCD takes a contiguous range of blocks, so neighbouring
+  // tiles share an L2
+  const long long it = xcd_remap(blockIdx.x, gridDim.x);
+  if (it >= g->blk0[g->nseg]) return;
+  const int s = group_seg(g, it, stw, 512);
+  const ColArgs& a = g->a[s];
+  const long long blk = it - g->blk0[s];
+  const int nbx = g->nbx[s], b = (int)(blk / nbx), bx = (int)(blk - (long long)b * nbx);
+  switch (a.nx) {
+    case 256: col_block<256, MODE, MUL, GroupCol<256>>(a, bx, b, nbx, lds, stw); break;
+    case 512: col_block<512, MODE, MUL, GroupCol<512>>(a, bx, b, nbx, lds, stw); break;
+    case 1024: col_block<1024, MODE, MUL, GroupCol<1024>>(a, bx, b, nbx, lds, stw); break;
+    default: col_block<2048, MODE, MUL, GroupCol<2048>>(a, bx, b, nbx, lds, stw); break;
+  }
+}
+
+static bool group_len(int n) { return n == 256 || n == 512 || n == 1024 || n == 2048; }
+
+template <int N> static int group_row_lines() { return RowCfg<N, GroupRow<N>::RV>::LINES; }
+static int group_row_lines(int n) {
+  return n == 256 ? group_row_lines<256>() : n == 512 ? group_row_lines<512>() : n == 1024 ? group_row_lines<1024>()
+                                                                                           : group_row_lines<2048>();
+}
+static int group_col_tile(int n) {
+  return n == 256 ? GroupCol<256>::C : n == 512 ? GroupCol<512>::C : n == 1024 ? GroupCol<1024>::C : GroupCol<2048>::C;
+}
+
+hipError_t build_row_group(const RowArgs* segs, int nseg, PassGroup<RowArgs>& g) {
+  if (nseg < 1 || nseg > kGroupMaxSeg) return hipErrorInvalidValue;
+  g = PassGroup<RowArgs>{};
+  int k = 0;
+  long long total = 0;
+  for (int i = 0; i < nseg; ++i) {
+    const RowArgs& a = segs[i];
+    if (!group_len(a.nx) || a.ny != a.nx || a.aux || a.prune_in != segs[0].prune_in ||
+        a.prune_out != segs[0].prune_out)
+      return hipErrorInvalidValue;
+    if (a.nbatch <= 0) continue;
+    g.a[k] = a;
+    g.blk0[k] = total;
+    const int lines = group_row_lines(a.nx);
+    total += ((long long)a.nbatch * a.ny + lines - 1) / lines;
+    ++k;
+  }
+  g.nseg = k;
+  g.blk0[k] = total;
+  return hipSuccess;
+}
+
+hipError_t build_col_group(const ColArgs* segs, int nseg, PassGroup<ColArgs>& g) {
+  if (nseg < 1 || nseg > kGroupMaxSeg) return hipErrorInvalidValue;
+  g = PassGroup<ColArgs>{};
+  int k = 0;
+  long long total = 0;
+  for (int i = 0; i < nseg; ++i) {
+    const ColArgs& a = segs[i];
+    if (!group_len(a.nx) || a.ny != a.nx || !a.prop_rows) return hipErrorInvalidValue;
+    if (a.nbatch <= 0) continue;
+    const int C = group_col_tile(a.nx);
+    g.a[k] = a;
+    // tiles_lo counts tiles of the default width; convert to this tile's
+    if (a.tiles_lo > 0) {
+      if ((a.nx / 4) % C != 0) return hipErrorInvalidValue;
+      g.a[k].tiles_lo = a.nx / 4 / C;
+    }
+    g.nbx[k] = g.a[k].tiles_lo > 0 ? 2 * g.a[k].tiles_lo : a.nx / C;
+    g.blk0[k] = total;
+    total += (long long)g.nbx[k] * a.nbatch;
+    ++k;
+  }
+  g.nseg = k;
+  g.blk0[k] = total;
+  return hipSuccess;
+}
+
+hipError_t launch_row_group(int mode, int mul, const PassGroup<RowArgs>& h, const PassGroup<RowArgs>* d,
+                            hipStream_t s) {
+  const long long total = h.blk0[h.nseg];
+  if (h.nseg == 0 || total <= 0) return hipSuccess;
+  if (!d) return hipErrorInvalidValue;
+  const int fl = (h.a[0].prune_in ? PRUNE_IN : 0) | (h.a[0].prune_out ? PRUNE_OUT : 0);
+  if (total > 0x7fffffffLL) return hipErrorInvalidValue;
+#define MSL_RG(M, U, F) hipLaunchKernelGGL((k_row_group<M, U, F>), dim3((unsigned)total), dim3(256), 0, s, d)
+  if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_RG(ROW_INV_MUL_FWD, RM_T, PRUNE_OUT);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT))
+    MSL_RG(ROW_INV_MUL_FWD, RM_T, PRUNE_IN | PRUNE_OUT);
+  else return hipErrorInvalidValue;
+#undef MSL_RG
+  return hipGetLastError();
+}
+
+hipError_t launch_col_group(int mode, int mul, const PassGroup<ColArgs>& h, const PassGroup<ColArgs>* d,
+                            hipStream_t s) {
+  const long long total = h.blk0[h.nseg];
+  if (h.nseg == 0 || total <= 0) return hipSuccess;
+  if (!d) return hipErrorInvalidValue;
+  if (total > 0x7fffffffLL) return hipErrorInvalidValue;
+#define MSL_CG(M, U) hipLaunchKernelGGL((k_col_group<M, U>), dim3((unsigned)total), dim3(512), 0, s, d)
+  if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_CG(COL_FWD_MUL_INV, CM_PROP);
+  else return hipErrorInvalidValue;
+#undef MSL_CG
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ dispatch
@@ -371,6 +809,56 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    cus[dev] = 256;
+  return cus[dev];
+}
+
+// persistent streamed column pass (variant 5), a full complement of resident workgroups
+template <int N>
+static hipError_t col_stream(int mode, int mul, const ColArgs& a, hipStream_t s) {
+  constexpr int C = ColStream<N>::C;
+  if (a.nbatch <= 0) return hipSuccess;
+  if (a.nx % C != 0 || a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / ColTile0<N>::C) return hipErrorInvalidValue;
+  // tiles_lo counts tiles of the default width; convert to this variant's
+  ColArgs b = a;
+  if (a.tiles_lo > 0) {
+    if ((a.nx / 4) % C != 0) return hipErrorInvalidValue;
+    b.tiles_lo = a.nx / 4 / C;
+  }
+  const int ntl = b.tiles_lo > 0 ? 2 * b.tiles_lo : a.nx / C;
+  const long long items = (long long)ntl * a.nbatch;
+  if (items > (1LL << 30)) return hipErrorInvalidValue;
+  const dim3 block(ColStream<N>::THREADS);
+  // as many resident workgroups as the LDS and registers of this instance allow
+#define MSL_COLS(M, U)                                                                                         \
+  do {                                                                                                         \
+    int per_cu = 0;                                                                                            \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_col_stream<N, M, U>,             \
+                                                     ColStream<N>::THREADS, 0) != hipSuccess || per_cu < 1)    \
+      per_cu = 1;                                                                                              \
+    const int grid = (int)std::min<long long>(items, (long long)per_cu * device_cus());                        \
+    hipLaunchKernelGGL((k_col_stream<N, M, U>), dim3(grid), block, 0, s, b, ntl, (int)items);                  \
+  } while (0)
+  if (mode == COL_FWD) MSL_COLS(COL_FWD, CM_SCALE);
+  else if (mode == COL_INV) MSL_COLS(COL_INV, CM_SCALE);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLS(COL_FWD_MUL_INV, CM_PROP);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLS(COL_FWD_MUL_INV, CM_BAND);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLS(COL_FWD_MUL_INV, CM_GRAD);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL_INV, CM_SCALE);
+  else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COLS(COL_FWD_MUL, CM_PROP);
+  else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL, CM_SCALE);
+  else if (mode == COL_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_MUL_INV, CM_SCALE);
+  else return hipErrorInvalidValue;
+#undef MSL_COLS
+  return hipGetLastError();
+}
+
 template <int N>
 static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
   // the detector pass reduces over the workgroup in an interleaved layout
@@ -378,6 +866,7 @@ static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int 
     if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
     return var == 2 ? col_nv<N, 2>(mode, mul, a, s) : col_nv<N, 0>(mode, mul, a, s);
   }
+  if (var == 5) return col_stream<N>(mode, mul, a, s);
   if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
   if (var == 3) return col_nv<N, 3>(mode, mul, a, s);
   if (var == 2) return col_nv<N, 2>(mode, mul, a, s);
@@ -399,7 +888,12 @@ hipError_t launch_row_v(int mode, int mul, const RowArgs& a, hipStream_t s, int 
   return hipErrorInvalidValue;
 }
 
+static bool needs_prop_rows(int mode, int mul) {
+  return (mul == CM_PROP || mul == CM_PROPCONJ) && mode != COL_FWD && mode != COL_INV;
+}
+
 hipError_t launch_col_v(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
+  if (needs_prop_rows(mode, mul) && !a.prop_rows) return hipErrorInvalidValue;
   switch (a.ny) {
     case 16: return col_n<16>(mode, mul, a, s, var);
     case 32: return col_n<32>(mode, mul, a, s, var);

@@ -9,7 +9,7 @@
 
 namespace msl {
 
-static inline unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline __host__ __device__ unsigned nblk(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ============================================================== binning
 // atombin2 (U.cu:277-303): bin ids from FP32 sub, div, mul(50), floor --
@@ -258,15 +258,15 @@ constexpr int ACAP = 1024;  // atoms staged in LDS per chunk
 // owns a 16x16 pixel tile of one slice; the atoms of every CSR bin within
 // sqrt(5) A of the tile (slab window per mode) are staged through LDS in
 // chunks, then each thread sums its pixel's contributions.
-__global__ void __launch_bounds__(256) k_potential(PotArgs a) {
+// pixel tile (tx, ty) of slice `slice` (relative to first_slice); 256 threads
+__device__ __forceinline__ void potential_tile(const PotArgs& a, int tx, int ty, int slice) {
   __shared__ float zt[103 * 24];
   __shared__ float sx[ACAP], sy[ACAP], sz[ACAP];
   __shared__ int sZ[ACAP];
   __shared__ int rs[64], rl[65];
   const int tid = threadIdx.x;
   for (int i = tid; i < 103 * 24; i += 256) zt[i] = a.ztab[i];
-  const int ix = blockIdx.x * PT + (tid % PT), iy = blockIdx.y * PT + (tid / PT);
-  const int slice = blockIdx.z;
+  const int ix = tx * PT + (tid % PT), iy = ty * PT + (tid / PT);
   float zf, h;
   {
 #pragma clang fp contract(off)
@@ -288,8 +288,8 @@ __global__ void __launch_bounds__(256) k_potential(PotArgs a) {
   const float px = pix_pos(ix, a.ps, offx), py = pix_pos(iy, a.ps, offy);
   // bin window of the tile: every bin that may hold an atom within sqrt(5)
   const float wx = (a.MaxX - a.MinX) / 50.f, wy = (a.MaxY - a.MinY) / 50.f, reach = 2.2360680f;
-  const float tx0 = offx + (float)(blockIdx.x * PT) * a.ps, tx1 = offx + (float)(blockIdx.x * PT + PT - 1) * a.ps;
-  const float ty0 = offy + (float)(blockIdx.y * PT) * a.ps, ty1 = offy + (float)(blockIdx.y * PT + PT - 1) * a.ps;
+  const float tx0 = offx + (float)(tx * PT) * a.ps, tx1 = offx + (float)(tx * PT + PT - 1) * a.ps;
+  const float ty0 = offy + (float)(ty * PT) * a.ps, ty1 = offy + (float)(ty * PT + PT - 1) * a.ps;
   const int i0 = max(0, (int)floorf((tx0 - reach) / wx) - 1), i1 = min(49, (int)floorf((tx1 + reach) / wx) + 1);
   const int j0 = max(0, (int)floorf((ty0 - reach) / wy) - 1), j1 = min(49, (int)floorf((ty1 + reach) / wy) + 1);
   const float h20 = a.dz / 20.0f;
@@ -365,10 +365,40 @@ __global__ void __launch_bounds__(256) k_potential(PotArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_potential(PotArgs a) { potential_tile(a, blockIdx.x, blockIdx.y, blockIdx.z); }
+
+// slice `first` of njobs jobs in one launch: workgroup id -> (job, tile)
+// through the block offsets blk0[0..njobs] (ascending)
+__global__ void __launch_bounds__(256) k_potential_group(const PotArgs* tab, const long long* blk0, int njobs,
+                                                         int first) {
+  const long long id = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (blk0[mid] <= id) lo = mid;
+    else hi = mid - 1;
+  }
+  PotArgs a = tab[lo];
+  a.first_slice = first;
+  const int tx = nblk(a.nx, PT);
+  const long long local = id - blk0[lo];
+  potential_tile(a, (int)(local % tx), (int)(local / tx), 0);
+}
+
 hipError_t launch_potential(const PotArgs& a, hipStream_t s) {
   if (a.nslices <= 0) return hipSuccess;
   dim3 grid(nblk(a.nx, PT), nblk(a.ny, PT), a.nslices);
   hipLaunchKernelGGL(k_potential, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+int potential_tiles(int nx, int ny) { return nblk(nx, PT) * nblk(ny, PT); }
+
+hipError_t launch_potential_group(const PotArgs* tab, const long long* blk0, int njobs, long long nblocks, int first,
+                                  hipStream_t s) {
+  if (njobs <= 0 || nblocks <= 0) return hipSuccess;
+  if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_potential_group, dim3((unsigned)nblocks), dim3(256), 0, s, tab, blk0, njobs, first);
   return hipGetLastError();
 }
 

@@ -54,6 +54,7 @@ struct msl_diff {
   std::vector<float> hkx, hky;
   Buf<float> kx, ky, partial;
   Buf<float2> twx, twy, px, py, W, G, exitw, ckpt, seg;
+  Buf<int2> prow;  // rows of each column inside |k| < kmax
   const float* V = nullptr;
   const float2* psi0 = nullptr;
   bool have_forward = false;
@@ -82,7 +83,7 @@ ColArgs cargs(msl_diff* d, const float2* src, float2* dst, float scale) {
   ColArgs a{};
   a.src = src; a.dst = dst; a.src_bstride = a.dst_bstride = d->np();
   a.nx = d->nx; a.ny = d->ny; a.nbatch = d->B; a.kx = d->kx.p; a.ky = d->ky.p; a.px = d->px.p; a.py = d->py.p;
-  a.kmax = d->kmax; a.scale = scale; a.tw = d->twy.p;
+  a.kmax = d->kmax; a.scale = scale; a.tw = d->twy.p; a.prop_rows = d->prow.p;
   a.tiles_lo = d->prune_tiles;
   return a;
 }
@@ -154,6 +155,8 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     };
     up(d->kx, d->hkx); up(d->ky, d->hky); up(d->twx, tw(nx)); up(d->twy, tw(ny));
     up(d->px, pr(d->hkx)); up(d->py, pr(d->hky));
+    d->prow.alloc(nx);
+    HC(make_prop_rows(d->kx.p, d->ky.p, nx, ny, d->kmax, d->prow.p, nullptr));
     const size_t wave = (size_t)nprobe * nx * ny;
     d->W.alloc(wave);
     d->G.alloc(wave);

@@ -141,6 +141,7 @@ ColArgs col_args(msl_sim* s, const float2* src, float2* dst, int nbatch, long lo
   a.ky = s->ky.p;
   a.px = s->px.p;
   a.py = s->py.p;
+  a.prop_rows = s->prow.p;
   a.kmax = s->kmax;
   a.scale = 1.0f;
   a.tw = s->twy.p;
@@ -333,6 +334,8 @@ void precalc(msl_sim* s) {
   s->ky.ensure(ry);
   HIPC(hipMemcpyAsync(s->kx.p, s->hkx.data(), rx * sizeof(float), hipMemcpyHostToDevice, s->stream));
   HIPC(hipMemcpyAsync(s->ky.p, s->hky.data(), ry * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  s->prow.ensure(rx);
+  HIPC(make_prop_rows(s->kx.p, s->ky.p, rx, ry, s->kmax, s->prow.p, s->stream));
   auto tx = twiddles(rx), ty = twiddles(ry);
   s->twx.ensure(rx);
   s->twy.ensure(ry);
@@ -352,6 +355,12 @@ void alloc_waves(msl_sim* s) {
 
 void potential(msl_sim* s, int mode, int first, int nslices, float dz, float zplane, int out_kind, void* out,
                long long sstride) {
+  const PotArgs p = pot_args(s, mode, first, nslices, dz, zplane, out_kind, out, sstride);
+  HIPC(launch_potential(p, s->stream));
+}
+
+PotArgs pot_args(msl_sim* s, int mode, int first, int nslices, float dz, float zplane, int out_kind, void* out,
+                 long long sstride) {
   require(s->binned, MSL_ERR_STATE, "UploadBinnedAtoms first");
   ensure_ztab(s);
   PotArgs p{};
@@ -362,7 +371,7 @@ void potential(msl_sim* s, int mode, int first, int nslices, float dz, float zpl
   p.dz = dz; p.first_slice = first; p.nslices = nslices; p.zplane = zplane;
   p.loadslicesz = s->loadslicesz; p.sigma = s->sigma2; p.mode = mode; p.out_kind = out_kind;
   p.out = out; p.out_sstride = sstride;
-  HIPC(launch_potential(p, s->stream));
+  return p;
 }
 
 // host argmax / argmin of |psi| with first-occurrence ties (cublasIsamax /

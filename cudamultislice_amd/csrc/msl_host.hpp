@@ -100,6 +100,7 @@ struct msl_sim {
   std::vector<float> hkx, hky;
   mslh::DBuf<float> kx, ky;
   mslh::DBuf<float2> twx, twy, px, py;
+  mslh::DBuf<int2> prow;  // rows of each column inside |k| < kmax
   float prop_dz = std::nanf("");
   float conventionaldz = 1.0f;
 
@@ -153,6 +154,9 @@ void precalc(msl_sim* s);
 // 2 real V), slices sstride elements apart
 void potential(msl_sim* s, int mode, int first, int nslices, float dz, float zplane, int out_kind, void* out,
                long long sstride);
+// the launch arguments of the same, without launching
+msl::PotArgs pot_args(msl_sim* s, int mode, int first, int nslices, float dz, float zplane, int out_kind, void* out,
+                      long long sstride);
 float download_scalar(msl_sim* s, const float* d);
 std::vector<float2> twiddles(int n);
 

@@ -49,6 +49,8 @@ struct ColArgs {
   const float* ky;
   const float2* px;       // separable propagator factors exp(-i pi lambda dz k^2)
   const float2* py;
+  const int2* prop_rows;  // CM_PROP / CM_PROPCONJ: column x is inside |k| < kmax at rows
+                          // r <= prop_rows[x].x and r >= prop_rows[x].y (launch_prop_rows)
   float kmax;
   float scale;
   const float2* tw;       // twiddles for length ny
@@ -63,6 +65,10 @@ struct ColArgs {
   int tiles_lo;
 };
 
+// rows of each column inside the propagator band |k| < kmax (nx entries,
+// device), built on s and waited for; hipErrorInvalidValue if some column's
+// band is not a run from each end of ky
+hipError_t make_prop_rows(const float* kx, const float* ky, int nx, int ny, float kmax, int2* rows, hipStream_t s);
 int col_tile(int ny);  // columns per column-pass workgroup (detector layout)
 int band_tiles_lo(int nx);  // column tiles per end covering x < N/4, x >= 3N/4 (0: no pruning)
 // pass layout variants per length (kind 0 row, 1 column), see k_pass.hip
@@ -77,6 +83,27 @@ hipError_t launch_hash_fill(float2* w, long long n, unsigned seed, int unit, hip
 hipError_t launch_copy16(const void* src, void* dst, long long bytes, hipStream_t s);
 
 hipError_t launch_row(int mode, int mul, const RowArgs& a, hipStream_t s);
+// One row / column launch over the waves of several grid sizes (one segment
+// per length 256, 512, 1024 or 2048; at most kGroupMaxSeg segments, square
+// grids).  Row passes: ROW_INV_MUL_FWD with RM_T, band-limited output, the
+// same prune flags in every segment.  Column passes: COL_FWD_MUL_INV with
+// CM_PROP.  Segments with nbatch 0 are skipped.
+// build_*_group validates the segments and lays out their blocks into a
+// table; the launch reads the table from device memory (d, a copy of h that
+// the caller uploaded) and sizes its persistent grid from h.
+constexpr int kGroupMaxSeg = 4;
+template <class A> struct PassGroup {
+  A a[kGroupMaxSeg];
+  long long blk0[kGroupMaxSeg + 1];  // first block of each segment; blk0[nseg] = total
+  int nbx[kGroupMaxSeg];             // columns: launched tiles per wave
+  int nseg;
+};
+hipError_t build_row_group(const RowArgs* segs, int nseg, PassGroup<RowArgs>& g);
+hipError_t build_col_group(const ColArgs* segs, int nseg, PassGroup<ColArgs>& g);
+hipError_t launch_row_group(int mode, int mul, const PassGroup<RowArgs>& h, const PassGroup<RowArgs>* d,
+                            hipStream_t s);
+hipError_t launch_col_group(int mode, int mul, const PassGroup<ColArgs>& h, const PassGroup<ColArgs>* d,
+                            hipStream_t s);
 hipError_t launch_col(int mode, int mul, const ColArgs& a, hipStream_t s);
 bool fft_size_supported(int n);
 
@@ -124,6 +151,12 @@ struct PotArgs {
   long long out_sstride;        // elements between slices
 };
 hipError_t launch_potential(const PotArgs& a, hipStream_t s);
+// one slice (first_slice = first, nslices 1) of several jobs in one launch:
+// tab[j] (device) holds job j's arguments, blk0[0..njobs] (device) the first
+// workgroup of each job, job j owning potential_tiles(nx, ny) workgroups
+int potential_tiles(int nx, int ny);
+hipError_t launch_potential_group(const PotArgs* tab, const long long* blk0, int njobs, long long nblocks, int first,
+                                  hipStream_t s);
 void kirkland_transform(const float* fparams, float* ztab);  // host: 103x12 -> per-Z constants
 
 // Probe in k-space (U.cu:354-383) for nprobe positions: batch b gets

@@ -11,6 +11,11 @@
 // job through a per-wave slot index.  A job's potentials are synthesised per
 // slice from its own binned atoms, as in MultisliceStepConv (U.cu:2085-2124)
 // or MultisliceStep (U.cu:2040-2083).
+//
+// Grouped mode (msl_serve_params.group_sizes): conventional jobs of every
+// grid size 256..2048 share their batches; per slice one potential launch
+// covers every running job and one row and one column launch every running
+// wave, whatever its size (one segment per size, k_pass.hip).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,6 +61,11 @@ struct Job {
 struct Item {
   int job, p0, cnt, nslices;
 };
+// an item of a grouped batch and its size class
+struct GItem {
+  int cls;
+  Item it;
+};
 
 struct Stats {
   double setup = 0, prop = 0, batches = 0, waves = 0, wave_slices = 0, launches = 0, peak = 0;
@@ -66,14 +76,19 @@ class Runner {
   Runner(msl_server* srv, const msl_serve_params& p, std::vector<Job>& jobs, Stats& st)
       : srv_(srv), p_(p), jobs_(jobs), st_(st) {}
 
-  // jobs of one class, sharing the tables of cs
-  void run_class(const std::vector<int>& ids, msl_sim* cs, bool fd) {
+  // switch the per-size state to the tables of cs
+  void use_class(msl_sim* cs, bool fd) {
     cs_ = cs;
     fd_ = fd;
     n_ = cs->resX;
     np_ = (long long)n_ * n_;
     ntiles_ = n_ / col_tile(n_);
     prune_ = !fd && cs->prune_tiles > 0;
+  }
+
+  // jobs of one class, sharing the tables of cs
+  void run_class(const std::vector<int>& ids, msl_sim* cs, bool fd) {
+    use_class(cs, fd);
     ensure_prop(cs, fd ? cs->dzFD : p_.dz);
     bool probes = false;
     for (int j : ids) probes = probes || jobs_[j].d->nprobe > 0;
@@ -115,7 +130,165 @@ class Runner {
     }
   }
 
+  // conventional jobs of several grid sizes (256..2048, band-pruned), one
+  // class per size; per slice one potential, one row and one column launch
+  // cover every running job / wave of every size
+  void run_group(const std::vector<std::pair<std::vector<int>, msl_sim*>>& classes) {
+    const long long budget = p_.budget_bytes > 0 ? p_.budget_bytes : (32LL << 30);
+    std::vector<GItem> items;
+    for (size_t c = 0; c < classes.size(); ++c) {
+      msl_sim* cs = classes[c].second;
+      use_class(cs, false);
+      require(prune_ && kGroupMaxSeg >= (int)classes.size(), MSL_ERR_UNSUPPORTED, "size class cannot be grouped");
+      ensure_prop(cs, p_.dz);
+      const long long wave_b = np_ * 8;
+      const long long maxw = std::max<long long>(1, (budget - wave_b) / (2 * wave_b));
+      for (int j : classes[c].first) {
+        const msl_job* d = jobs_[j].d;
+        if (d->nprobe <= 0) {
+          items.push_back({(int)c, {j, 0, 1, d->nslices}});
+        } else {
+          for (int p0 = 0; p0 < d->nprobe; p0 += (int)maxw)
+            items.push_back({(int)c, {j, p0, (int)std::min<long long>(maxw, d->nprobe - p0), d->nslices}});
+        }
+      }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const GItem& a, const GItem& b) { return a.it.nslices > b.it.nslices; });
+    size_t i0 = 0;
+    while (i0 < items.size()) {
+      long long bytes = 0;
+      size_t i1 = i0;
+      std::vector<int> seen;
+      while (i1 < items.size()) {
+        const msl_sim* cs = classes[items[i1].cls].second;
+        const long long np = (long long)cs->resX * cs->resX;
+        const bool newjob = std::find(seen.begin(), seen.end(), items[i1].it.job) == seen.end();
+        const long long add = items[i1].it.cnt * np * 8 + (newjob ? np * 8 : 0);
+        if (i1 > i0 && bytes + add > budget) break;
+        bytes += add;
+        if (newjob) seen.push_back(items[i1].it.job);
+        ++i1;
+      }
+      st_.peak = std::max(st_.peak, (double)bytes);
+      run_group_batch(classes, std::vector<GItem>(items.begin() + i0, items.begin() + i1));
+      i0 = i1;
+    }
+  }
+
  private:
+  void run_group_batch(const std::vector<std::pair<std::vector<int>, msl_sim*>>& classes,
+                       const std::vector<GItem>& batch) {
+    const int nc = (int)classes.size();
+    std::vector<Seg> seg(nc);
+    for (const GItem& g : batch) seg[g.cls].items.push_back(g.it);
+    // job slots are per size (each size's transmissions in its own array); a
+    // global job order, longest first, makes the running jobs of every slice
+    // a prefix of the potential table
+    std::vector<std::pair<int, int>> jorder;  // (class, job), longest first
+    for (int c = 0; c < nc; ++c) {
+      Seg& S = seg[c];
+      S.cs = classes[c].second;
+      if (S.items.empty()) continue;
+      use_class(S.cs, false);
+      bool probes = false;
+      for (const Item& it : S.items) probes = probes || jobs_[it.job].d->nprobe > 0;
+      if (probes) {
+        probe_norm();
+        S.norm = norm_;
+      }
+      scratch_.ensure(np_);
+      layout(S.items, S.L);
+      for (int j : S.L.jobs) jorder.push_back({c, j});
+      S.W.ensure((size_t)S.L.nw * np_);
+      S.T.ensure(S.L.jobs.size() * (size_t)np_);
+      S.partial.ensure((size_t)S.L.nw * ntiles_);
+      S.adf.ensure(S.L.nw);
+      for (size_t k = 0; k < S.items.size(); ++k) {
+        if (jobs_[S.items[k].job].d->nprobe > 0) continue;
+        float2* w = S.W.p + (size_t)S.L.w0[k] * np_;
+        HIPC(launch_fill(w, np_, make_float2(1.f, 0.f), st()));
+        RowArgs r = row_args(S.cs, w, w, 1, np_);
+        r.scale = 1.0f / (float)n_;
+        row(S.cs, ROW_FWD, RM_NONE, r);
+      }
+      norm_ = S.norm;
+      init_probes_kxy(S.items, S.L, S.W.p);
+    }
+    std::stable_sort(jorder.begin(), jorder.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+      return jobs_[a.second].d->nslices > jobs_[b.second].d->nslices;
+    });
+    // potential table: job k of jorder writes its size's transmission slot
+    const int nj = (int)jorder.size();
+    std::vector<PotArgs> ptab(nj);
+    std::vector<long long> pblk(nj + 1, 0);
+    for (int k = 0; k < nj; ++k) {
+      Seg& S = seg[jorder[k].first];
+      Job& jb = jobs_[jorder[k].second];
+      const long long np = (long long)S.cs->resX * S.cs->resX;
+      ptab[k] = pot_args(jb.s.get(), jb.s->potmode, 0, 1, p_.dz, 0.f, 0, S.T.p + (size_t)jb.slot * np, np);
+      pblk[k + 1] = pblk[k] + potential_tiles(S.cs->resX, S.cs->resX);
+    }
+    const int ns = batch.front().it.nslices;
+    // per-slice pass tables: the running waves of every size
+    std::vector<PassGroup<RowArgs>> rg(ns);
+    std::vector<PassGroup<ColArgs>> cg(ns);
+    for (int i = 1; i <= ns; ++i) {
+      RowArgs ra[kGroupMaxSeg];
+      ColArgs ca[kGroupMaxSeg];
+      int k = 0;
+      for (int c = 0; c < nc; ++c) {
+        Seg& S = seg[c];
+        const long long np = (long long)S.cs->resX * S.cs->resX;
+        const int A = S.items.empty() ? 0 : active_waves(S.items, S.L, i);
+        ra[k] = row_args(S.cs, S.W.p, S.W.p, A, np);
+        ra[k].t = S.T.p;
+        ra[k].t_index = S.L.tidx.p;
+        ra[k].t_bstride = np;
+        ra[k].prune_in = i > 1;
+        ra[k].prune_out = 1;
+        ca[k] = col_args(S.cs, S.W.p, S.W.p, A, np);
+        ca[k].scale = S.cs->nf * S.cs->nf;
+        ca[k].tiles_lo = S.cs->prune_tiles;
+        ++k;
+      }
+      HIPC(build_row_group(ra, k, rg[i - 1]));
+      HIPC(build_col_group(ca, k, cg[i - 1]));
+    }
+    DBuf<PotArgs> dptab;
+    DBuf<long long> dpblk;
+    DBuf<PassGroup<RowArgs>> drg;
+    DBuf<PassGroup<ColArgs>> dcg;
+    dptab.ensure(nj);
+    dpblk.ensure(nj + 1);
+    drg.ensure(ns);
+    dcg.ensure(ns);
+    HIPC(hipMemcpyAsync(dptab.p, ptab.data(), nj * sizeof(PotArgs), hipMemcpyHostToDevice, st()));
+    HIPC(hipMemcpyAsync(dpblk.p, pblk.data(), (nj + 1) * sizeof(long long), hipMemcpyHostToDevice, st()));
+    HIPC(hipMemcpyAsync(drg.p, rg.data(), ns * sizeof(PassGroup<RowArgs>), hipMemcpyHostToDevice, st()));
+    HIPC(hipMemcpyAsync(dcg.p, cg.data(), ns * sizeof(PassGroup<ColArgs>), hipMemcpyHostToDevice, st()));
+    HIPC(hipStreamSynchronize(st()));  // host tables may go once copied
+    const double t0 = now_ms();
+    for (int i = 1; i <= ns; ++i) {
+      int ja = 0;
+      while (ja < nj && jobs_[jorder[ja].second].d->nslices >= i) ++ja;
+      HIPC(launch_potential_group(dptab.p, dpblk.p, ja, pblk[ja], i, st()));
+      HIPC(launch_row_group(ROW_INV_MUL_FWD, RM_T, rg[i - 1], drg.p + (i - 1), st()));
+      HIPC(launch_col_group(COL_FWD_MUL_INV, CM_PROP, cg[i - 1], dcg.p + (i - 1), st()));
+      st_.launches += 3;
+      for (int c = 0; c < nc; ++c) {
+        Seg& S = seg[c];
+        if (S.items.empty()) continue;
+        st_.wave_slices += active_waves(S.items, S.L, i);
+        use_class(S.cs, false);
+        for (size_t k = 0; k < S.items.size(); ++k)
+          if (S.items[k].nslices == i) finish_conv(S.items, S.L, k, S.W.p, S.partial.p, S.adf.p);
+      }
+    }
+    HIPC(hipStreamSynchronize(st()));
+    st_.prop += now_ms() - t0;
+  }
+
   msl_server* srv_;
   const msl_serve_params& p_;
   std::vector<Job>& jobs_;
@@ -159,9 +332,19 @@ class Runner {
     int nw = 0;
     DBuf<int> tidx, pos;
   };
+  // one size inside a grouped batch
+  struct Seg {
+    msl_sim* cs = nullptr;
+    std::vector<Item> items;  // longest first
+    Layout L;
+    DBuf<float2> W, T;
+    DBuf<float> partial, adf;
+    float norm = 1.f;
+  };
+
   void layout(const std::vector<Item>& b, Layout& L) {
     L.w0.resize(b.size());
-    for (auto& j : jobs_) j.slot = -1;
+    for (const Item& it : b) jobs_[it.job].slot = -1;
     std::vector<int> tidx, px, py;
     for (size_t k = 0; k < b.size(); ++k) {
       Job& J = jobs_[b[k].job];
@@ -245,7 +428,6 @@ class Runner {
     }
     init_probes_kxy(b, L, W.p);
     const double t0 = now_ms();
-    const int det_tiles = prune_ ? 2 * cs_->prune_tiles : ntiles_;
     for (int i = 1; i <= b.front().nslices; ++i) {
       const int A = active_waves(b, L, i);
       for (int j : L.jobs) {
@@ -266,31 +448,36 @@ class Runner {
       st_.launches += 2;
       st_.wave_slices += A;
       // items ending here: detector (probes) or exit wave (plane wave)
-      for (size_t k = 0; k < b.size(); ++k) {
-        if (b[k].nslices != i) continue;
-        const Job& jb = jobs_[b[k].job];
-        float2* w = W.p + (size_t)L.w0[k] * np_;
-        if (jb.d->nprobe > 0) {
-          // unitary |Psi|^2 = |F_y z|^2 for z in the (kx, y) domain
-          ColArgs dc = col_args(cs_, w, nullptr, b[k].cnt, np_);
-          dc.kin = p_.adf_inner * 1e-3f / cs_->wavel;
-          dc.kout = p_.adf_outer * 1e-3f / cs_->wavel;
-          dc.det_scale = 1.0f;
-          dc.partial = partial.p + (size_t)L.w0[k] * det_tiles;
-          dc.tiles_lo = prune_ ? cs_->prune_tiles : 0;
-          col(cs_, COL_FWD_DETECT, CM_SCALE, dc);
-          HIPC(launch_reduce_rows(dc.partial, b[k].cnt, det_tiles, adf.p + L.w0[k], 0, st()));
-          adf_out(b, k, adf.p + L.w0[k]);
-        } else if (jb.d->exit_out) {
-          RowArgs ri = row_args(cs_, w, scratch_.p, 1, np_);
-          ri.prune_in = prune_;
-          row(cs_, ROW_INV, RM_NONE, ri);
-          HIPC(hipMemcpyAsync(jb.d->exit_out, scratch_.p, np_ * sizeof(float2), hipMemcpyDeviceToHost, st()));
-        }
-      }
+      for (size_t k = 0; k < b.size(); ++k)
+        if (b[k].nslices == i) finish_conv(b, L, k, W.p, partial.p, adf.p);
     }
     HIPC(hipStreamSynchronize(st()));
     st_.prop += now_ms() - t0;
+  }
+
+  // item k of a conventional batch after its last slice: detector (probes)
+  // or exit wave (plane wave); W in the (kx, y) domain
+  void finish_conv(const std::vector<Item>& b, const Layout& L, size_t k, float2* W, float* partial, float* adf) {
+    const int det_tiles = prune_ ? 2 * cs_->prune_tiles : ntiles_;
+    const Job& jb = jobs_[b[k].job];
+    float2* w = W + (size_t)L.w0[k] * np_;
+    if (jb.d->nprobe > 0) {
+      // unitary |Psi|^2 = |F_y z|^2 for z in the (kx, y) domain
+      ColArgs dc = col_args(cs_, w, nullptr, b[k].cnt, np_);
+      dc.kin = p_.adf_inner * 1e-3f / cs_->wavel;
+      dc.kout = p_.adf_outer * 1e-3f / cs_->wavel;
+      dc.det_scale = 1.0f;
+      dc.partial = partial + (size_t)L.w0[k] * det_tiles;
+      dc.tiles_lo = prune_ ? cs_->prune_tiles : 0;
+      col(cs_, COL_FWD_DETECT, CM_SCALE, dc);
+      HIPC(launch_reduce_rows(dc.partial, b[k].cnt, det_tiles, adf + L.w0[k], 0, st()));
+      adf_out(b, k, adf + L.w0[k]);
+    } else if (jb.d->exit_out) {
+      RowArgs ri = row_args(cs_, w, scratch_.p, 1, np_);
+      ri.prune_in = prune_;
+      row(cs_, ROW_INV, RM_NONE, ri);
+      HIPC(hipMemcpyAsync(jb.d->exit_out, scratch_.p, np_ * sizeof(float2), hipMemcpyDeviceToHost, st()));
+    }
   }
 
   // finite-difference multislice (U.cu:2040-2083) over one batch
@@ -464,10 +651,18 @@ msl_status msl_server_run(msl_server* srv, const msl_serve_params* prm, const ms
     }
     st.setup = now_ms() - t0;
     Runner R(srv, *prm, jobs, st);
+    std::vector<std::pair<std::vector<int>, msl_sim*>> grouped;
     for (auto& kv : classes) {
       const int first = kv.second.front();
-      R.run_class(kv.second, jobs[first].s.get(), std::get<1>(kv.first) == MSL_METHOD_FD);
+      msl_sim* cs = jobs[first].s.get();
+      const int n = std::get<0>(kv.first);
+      const bool fd = std::get<1>(kv.first) == MSL_METHOD_FD;
+      if (prm->group_sizes && !fd && cs->prune_tiles > 0 && (n == 256 || n == 512 || n == 1024 || n == 2048))
+        grouped.push_back({kv.second, cs});
+      else
+        R.run_class(kv.second, cs, fd);
     }
+    if (!grouped.empty()) R.run_group(grouped);
     HIPC(hipStreamSynchronize(srv->stream));
     if (stats) {
       stats[0] = now_ms() - t0; stats[1] = st.setup; stats[2] = st.prop; stats[3] = st.batches;

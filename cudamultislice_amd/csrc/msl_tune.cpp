@@ -27,12 +27,12 @@ struct Dev {
 struct Bench {
   int n, nb;
   std::vector<float> kx;
-  Dev W, t, k, pxy, tw;
+  Dev W, t, k, pxy, tw, rows;
   float kmax;
   Bench(int n_, int nb_)
       : n(n_), nb(nb_), W((size_t)nb_ * n_ * n_ * 8), t((size_t)n_ * n_ * 8), k((size_t)n_ * 4),
-        pxy((size_t)n_ * 8), tw((size_t)n_ * 8) {}
-  bool ok() const { return W.p && t.p && k.p && pxy.p && tw.p; }
+        pxy((size_t)n_ * 8), tw((size_t)n_ * 8), rows((size_t)n_ * 8) {}
+  bool ok() const { return W.p && t.p && k.p && pxy.p && tw.p && rows.p; }
   hipError_t init(hipStream_t s) {
     // frequencies / propagator of a 0.1 A grid at 300 kV, 2 A slices
     std::vector<float> kk(n);
@@ -51,6 +51,7 @@ struct Bench {
     if ((e = hipMemcpyAsync(pxy.p, p.data(), n * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(tw.p, w.data(), n * 8, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
     if ((e = launch_hash_fill(t.as<float2>(), (long long)n * n, 7u, 1, s)) != hipSuccess) return e;
+    if ((e = make_prop_rows(k.as<float>(), k.as<float>(), n, n, kmax, rows.as<int2>(), s)) != hipSuccess) return e;
     return hipStreamSynchronize(s);
   }
   hipError_t fill(float2* w, int nw, hipStream_t s) { return launch_hash_fill(w, (long long)nw * n * n, 11u, 0, s); }
@@ -69,7 +70,7 @@ struct Bench {
     ColArgs c{};
     c.src = w; c.dst = w; c.src_bstride = c.dst_bstride = bs; c.nx = n; c.ny = n; c.nbatch = nw;
     c.kx = k.as<float>(); c.ky = k.as<float>(); c.px = pxy.as<float2>(); c.py = pxy.as<float2>();
-    c.kmax = kmax; c.scale = 1.0f / ((float)n * n); c.tw = tw.as<float2>();
+    c.kmax = kmax; c.scale = 1.0f / ((float)n * n); c.tw = tw.as<float2>(); c.prop_rows = rows.as<int2>();
     c.tiles_lo = tiles_lo;
     return launch_col_v(COL_FWD_MUL_INV, CM_PROP, c, s, var);
   }
@@ -157,13 +158,13 @@ msl_status msl_bench_pass(int kind, int variant, int n, int nbatch, int reps, fl
 
 // Autotuner: for length n and a batch of nbatch waves, time every row and
 // column variant that agrees with variant 0 to 1e-5 and select the fastest.
-// out (optional): [0..4] row ms per variant, [5..9] column ms per variant,
+// out (optional): [0..5] row ms per variant, [6..11] column ms per variant,
 // -1 for variants rejected or not built.
 msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* col_var, float* out) {
   if (!fft_size_supported(n) || nbatch < 1 || reps < 1) return MSL_ERR_ARG;
   int best[2] = {0, 0};
   if (out)
-    for (int i = 0; i < 10; ++i) out[i] = -1.f;
+    for (int i = 0; i < 12; ++i) out[i] = -1.f;
   for (int kind = 0; kind < 2; ++kind) {
     float bms = 1e30f;
     for (int v = 0; v < num_pass_variants(kind); ++v) {
@@ -171,7 +172,7 @@ msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* c
       msl_status st = msl_bench_pass(kind, v, n, nbatch, reps, &ms, &md);
       if (st != MSL_OK) return st;
       if (!(md < 1e-5f)) ms = -1;
-      if (out && v < 5) out[5 * kind + v] = ms;
+      if (out && v < 6) out[6 * kind + v] = ms;
       if (ms > 0 && ms < bms) {
         bms = ms;
         best[kind] = v;

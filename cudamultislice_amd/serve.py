@@ -45,9 +45,11 @@ def _p(a):
 
 
 def serve(jobs: list[Job], kV: float = 300.0, ps: float = 0.1, dz: float = 2.0, aperture: float = 25.0,
-          adf=(60.0, 200.0), potential_mode: int = 0, budget_bytes: int = 0, device: int = 0) -> dict:
+          adf=(60.0, 200.0), potential_mode: int = 0, budget_bytes: int = 0, device: int = 0,
+          group_sizes: bool = True) -> dict:
     """Runs every job; fills job.result ("exit" for plane waves, "adf" for
-    probes) and returns the engine's statistics."""
+    probes) and returns the engine's statistics.  group_sizes: conventional
+    jobs of grid 256..2048 share one launch per pass and slice."""
     L = lib()
     keep = []
     arr = (msl_job * len(jobs))()
@@ -70,7 +72,7 @@ def serve(jobs: list[Job], kV: float = 300.0, ps: float = 0.1, dz: float = 2.0, 
             ew = np.zeros((j.n, j.n), np.complex64)
             d.exit_out = _p(ew)
             j.result["exit"] = ew
-    prm = msl_serve_params(kV, ps, dz, aperture, adf[0], adf[1], potential_mode, budget_bytes)
+    prm = msl_serve_params(kV, ps, dz, aperture, adf[0], adf[1], potential_mode, budget_bytes, int(group_sizes))
     h = C.c_void_p()
     st = L.msl_server_create(C.byref(h), device)
     if st != 0:

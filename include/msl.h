@@ -224,7 +224,7 @@ msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, flo
  * k_pass.hip.  msl_bench_pass times `reps` launches on an n x n batch of
  * nbatch waves and, with maxdiff, compares against variant 0;
  * msl_autotune_passes selects the fastest agreeing variant per pass for n
- * (out[10]: [0..4] row and [5..9] column timings in ms per variant, -1 where
+ * (out[12]: [0..5] row and [6..11] column timings in ms per variant, -1 where
  * rejected or absent).  Timings use band-pruned passes where the length allows. */
 msl_status msl_set_pass_variant(int kind, int n, int variant);
 msl_status msl_get_pass_variant(int kind, int n, int* variant);
@@ -282,6 +282,9 @@ typedef struct {
   float adf_inner, adf_outer; /* mrad */
   int potential_mode;     /* MSL_POT_* */
   long long budget_bytes; /* device memory for the waves of one batch (0: 32 GiB) */
+  int group_sizes;        /* 1: conventional jobs of grid 256..2048 share batches, with one
+                             potential, one row and one column launch per slice for all
+                             sizes; 0: every size class runs its own launches */
 } msl_serve_params;
 typedef struct msl_server msl_server;
 msl_status msl_server_create(msl_server** out, int device);

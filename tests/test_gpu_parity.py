@@ -427,7 +427,7 @@ def test_every_pass_variant_matches_oracle_in_batched_stem(fp):
     o.bin()
     want = o.stem_batch(px, py, dz, ns, 10.0, 200.0)["adf"]
     L = lib()
-    nrow, ncol = 3, 5
+    nrow, ncol = 3, 6
     try:
         for rv in range(nrow):
             for cv in range(ncol):
@@ -548,6 +548,50 @@ def test_serving_mix_of_uneven_jobs_matches_oracle(fp):
     for a, b in zip(jobs, again):
         for k in a.result:
             assert np.array_equal(a.result[k], b.result[k]), k
+
+
+def test_grouped_sizes_share_launches_and_match_oracle(fp):
+    """Conventional jobs of grid 256, 512, 1024 and 2048 in grouped batches
+    (one potential, row and column launch per slice for every size) agree
+    with the per-size launches to rounding (same plans and tiles; the
+    compiler may fuse multiply-adds differently in the two kernels) and match
+    the oracle job by job (2048 against the per-size run only); a 64-point
+    job beside them runs per size."""
+    from cudamultislice_amd.serve import Job, serve
+    kV, ps, dz, ap, adf = 300.0, 0.2, 2.0, 25.0, (10.0, 200.0)
+    si = S.srtio3(2, 2, 2)
+    ra = S.random_atoms(60, (8.0, 8.0, 6.0), seed=S.SEED + 5)
+    rb = S.random_atoms(120, (30.0, 30.0, 6.0), seed=S.SEED + 6)
+    pr = lambda *v: np.array(v, np.int32)  # noqa: E731
+
+    def mix():
+        return [
+            Job(256, *ra, nslices=4, posx=pr(3, 200, 255), posy=pr(7, 100, 0)),
+            Job(256, *si, nslices=6),
+            Job(512, *rb, nslices=3, posx=pr(0, 300), posy=pr(64, 9)),
+            Job(1024, *si, nslices=2),
+            Job(2048, *ra, nslices=2),
+            Job(64, *si, nslices=5),
+        ]
+    grouped, per_size = mix(), mix()
+    sg = serve(grouped, kV=kV, ps=ps, dz=dz, aperture=ap, adf=adf, group_sizes=True)
+    sp = serve(per_size, kV=kV, ps=ps, dz=dz, aperture=ap, adf=adf, group_sizes=False)
+    assert sg["launches"] < sp["launches"] and sg["wave_slices"] == sp["wave_slices"]
+    for a, b in zip(grouped, per_size):
+        for k in a.result:
+            if k == "adf":
+                assert np.max(np.abs(a.result[k] - b.result[k]) / b.result[k]) < 1e-5, (a.n, k)
+            else:
+                assert rel_l2(a.result[k], b.result[k]) < 1e-5, (a.n, k)
+    for j in grouped:
+        if j.n == 2048:
+            assert np.isfinite(j.result["exit"]).all()
+            continue
+        want = _serve_oracle(j, fp, kV, ps, dz, ap, adf)
+        if "exit" in want:
+            assert rel_l2(j.result["exit"], want["exit"]) < 1e-4, j.n
+        else:
+            assert np.max(np.abs(j.result["adf"] - want["adf"]) / want["adf"]) < 1e-4, j.n
 
 
 def test_diff_forward_and_gradient_match_fp64_adjoint():
