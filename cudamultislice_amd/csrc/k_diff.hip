@@ -51,10 +51,14 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
   const LL L{lds + ln * LS, 1};
   const float* vr = a.V + (long long)y * N;
   float2 tr[E];
+  // every potential value loaded before the first sincosf (whose range
+  // reduction branches would serialise the loads)
+#pragma unroll
+  for (int c = 0; c < E; ++c) tr[c].x = vr[t + c * T];
 #pragma unroll
   for (int c = 0; c < E; ++c) {
     float sn, cs;
-    sincosf(a.sigma * vr[t + c * T], &sn, &cs);
+    sincosf(a.sigma * tr[c].x, &sn, &cs);
     tr[c] = make_float2(cs, sn);
   }
   float acc[E];
@@ -64,15 +68,23 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
     const int p = (g * a.per_line + q) * LINES + ln;
     const bool ok = p < a.nbatch;
     float2 gu[E], w[E];
-    const long long ro = (long long)p * a.bstride + (long long)y * N;
+    // lines past the batch read wave 0 and are zeroed after the loads:
+    // unconditional loads issue together instead of one branch each
+    const long long ro = (long long)(ok ? p : 0) * a.bstride + (long long)y * N;
+    const float2 zero = make_float2(0.f, 0.f);
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const bool band = !PRUNE || band_block<E>(c);
-      gu[c] = (ok && band) ? a.G[ro + t + c * T] : make_float2(0.f, 0.f);
+      gu[c] = band ? a.G[ro + t + c * T] : zero;
       // the stored real-space waves are read once: past the caches, so the
       // gradient rows stay resident for the next column pass
-      if constexpr (REAL_Z) w[c] = ok ? nt_load(&a.Z[ro + t + c * T]) : make_float2(0.f, 0.f);
-      else w[c] = (ok && band) ? a.Z[ro + t + c * T] : make_float2(0.f, 0.f);
+      if constexpr (REAL_Z) w[c] = nt_load(&a.Z[ro + t + c * T]);
+      else w[c] = band ? a.Z[ro + t + c * T] : zero;
+    }
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      gu[c] = ok ? gu[c] : zero;
+      w[c] = ok ? w[c] : zero;
     }
     fft_line<Plan<N>, N, +1, S>(gu, t, stw, L);              // G_u = IF_x(G_a)
     if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S>(w, t, stw, L);  // w = IF_x(z_{i-1})

@@ -22,6 +22,19 @@
 
 namespace msl {
 
+// the launch structures as device code reads them from tables in memory
+// (pointers in the global address space, same layout)
+struct RowArgsG {
+  MSL_ROW_ARGS(MSL_GLOBAL)
+};
+struct ColArgsG {
+  MSL_COL_ARGS(MSL_GLOBAL)
+};
+static_assert(sizeof(RowArgsG) == sizeof(RowArgs) && sizeof(ColArgsG) == sizeof(ColArgs), "layouts differ");
+template <class A> struct GlobalView;
+template <> struct GlobalView<RowArgs> { using T = RowArgsG; };
+template <> struct GlobalView<ColArgs> { using T = ColArgsG; };
+
 static int lg2(int n) {
   int l = 0;
   while ((1 << l) < n) ++l;
@@ -46,13 +59,12 @@ static int lg2(int n) {
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 1-4 GiB
-// of waves): rows of 512 take the alternative plan (1.04x over 1); at 1024 the
+// of waves): rows of 512 take the alternative plan (1.08x over 1); at 1024 the
 // alternative plan holds 256 VGPRs (one wave per SIMD) when the row pass
-// evaluates exp(i sigma V) itself, so 1024 keeps 1.  Columns: 256 the
-// alternative plan (1.07x over 0), 512 half-width tiles (1.05x), 1024 two
-// LDS copies (1.03x), 2048 variant 0.
+// evaluates exp(i sigma V) itself, so 1024 keeps 1.  Columns: 512 and 1024
+// half-width tiles (1.06x, 1.03x over 0), 256 and 2048 variant 0.
 static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 1, 1};
-static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 2, 3, 4, 0};
+static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
 void set_pass_variant(int kind, int n, int v) {
@@ -141,8 +153,8 @@ template <int E> struct BandBlocks {
 
 // the LINES lines of workgroup-sized block blk of a row pass; lds holds
 // LINES * LS elements, tw the twiddles of length N
-template <int N, int MODE, int MUL, int FL, int RV>
-__device__ __forceinline__ void row_block(const RowArgs& a, long long blk, float2* lds, const float2* stw) {
+template <int N, int MODE, int MUL, int FL, int RV, class A>
+__device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds, const float2* stw) {
   using Cfg = RowCfg<N, RV>;
   using P = typename Cfg::P;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
@@ -182,10 +194,16 @@ __device__ __forceinline__ void row_block(const RowArgs& a, long long blk, float
   } else if constexpr (MUL == RM_PHASE) {
     if (ok) {
       const float* vr = a.V + (long long)(b / a.t_group) * a.t_bstride + (long long)y * N;
+      // every potential value loaded before the first sincosf: its range
+      // reduction branches would otherwise put one memory round trip
+      // between consecutive elements
+      float ph[E];
+#pragma unroll
+      for (int c = 0; c < E; ++c) ph[c] = vr[t + c * T];
 #pragma unroll
       for (int c = 0; c < E; ++c) {
         float sn, cs;
-        sincosf(a.sigma * vr[t + c * T], &sn, &cs);
+        sincosf(a.sigma * ph[c], &sn, &cs);
         v[c] = cscale(cmul(v[c], make_float2(cs, sn)), a.scale);
       }
     }
@@ -211,14 +229,16 @@ __device__ __forceinline__ void row_block(const RowArgs& a, long long blk, float
 // once instead of once per wave (a 2048^2 slice, 32 MiB, does not stay in an
 // XCD's L2 between one wave and the next).
 constexpr int kRowWaves = 8;
-__device__ __forceinline__ long long row_order(long long h, long long total, int nbatch, int ybs) {
-  const long long r = xcd_remap(h, total);
+__device__ __forceinline__ long long row_interleave(long long r, int nbatch, int ybs) {
   const long long span = (long long)kRowWaves * ybs;
   const long long full = (long long)(nbatch / kRowWaves) * span;  // blocks of complete wave groups
   if (r >= full) return r;
   const long long grp = r / span, rem = r - grp * span;
   const int yb = (int)(rem / kRowWaves), bi = (int)(rem - (long long)yb * kRowWaves);
   return (grp * kRowWaves + bi) * ybs + yb;
+}
+__device__ __forceinline__ long long row_order(long long h, long long total, int nbatch, int ybs) {
+  return row_interleave(xcd_remap(h, total), nbatch, ybs);
 }
 
 // The kernel keeps its own copy of the row_block body, on its own LDS arrays:
@@ -258,7 +278,8 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
     }
   }
   const LL L{lds + ln * LS, 1};
-  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, stw, L);
+  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD || MODE == ROW_INV_FD_FWD)
+    fft_line<P, N, +1, S>(v, t, stw, L);
   if constexpr ((FL & STORE_AUX) != 0) {
     // read back only much later (by the adjoint pass): streamed past the caches
     if (ok) {
@@ -267,7 +288,34 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
       for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
     }
   }
-  if constexpr (MUL == RM_T) {
+  if constexpr (MODE == ROW_INV_FD_FWD) {
+    // finiteDifferenceKernel, U.cu:995-1023: v holds the Laplacian G of psi
+    if (ok) {
+      const long long o = (long long)b * a.dst_bstride + (long long)y * N;
+      const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
+      const float2* vr = a.t + slot * a.t_bstride + (long long)y * N;
+      float2 pm[E], p0[E];
+      float vv[E];
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        pm[c] = a.fd_pm[o + t + c * T];
+        p0[c] = a.fd_p0[o + t + c * T];
+        vv[c] = vr[t + c * T].x;
+      }
+      const float dz = a.fd_dz, wavel = a.fd_wavel;
+      const float al = 2.f * 3.14159f * dz / wavel;
+      const float den = 1.f + al * al;
+      const float2 rcp = make_float2(1.f / den, -al / den);                    // 1 / (1 + i a)
+      const float2 ratio = make_float2((1.f - al * al) / den, -2.f * al / den);  // (1 - i a) / (1 + i a)
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        const float2 p = p0[c], g = v[c];
+        const float w = dz * dz * 4.f * 3.14159f * a.sigma * vv[c] / wavel;
+        const float2 A = make_float2(2.f * p.x - dz * dz * g.x - w * p.x, 2.f * p.y - dz * dz * g.y - w * p.y);
+        v[c] = csub(cmul(rcp, A), cmul(ratio, pm[c]));
+      }
+    }
+  } else if constexpr (MUL == RM_T) {
     if (ok) {
       const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
@@ -277,10 +325,16 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
   } else if constexpr (MUL == RM_PHASE) {
     if (ok) {
       const float* vr = a.V + (long long)(b / a.t_group) * a.t_bstride + (long long)y * N;
+      // every potential value loaded before the first sincosf: its range
+      // reduction branches would otherwise put one memory round trip
+      // between consecutive elements
+      float ph[E];
+#pragma unroll
+      for (int c = 0; c < E; ++c) ph[c] = vr[t + c * T];
 #pragma unroll
       for (int c = 0; c < E; ++c) {
         float sn, cs;
-        sincosf(a.sigma * vr[t + c * T], &sn, &cs);
+        sincosf(a.sigma * ph[c], &sn, &cs);
         v[c] = cscale(cmul(v[c], make_float2(cs, sn)), a.scale);
       }
     }
@@ -290,7 +344,8 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
       for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
     }
   }
-  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
+  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD || MODE == ROW_INV_FD_FWD)
+    fft_line<P, N, -1, S>(v, t, stw, L);
   if (ok) {
     float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
@@ -341,20 +396,22 @@ hipError_t make_prop_rows(const float* kx, const float* ky, int nx, int ny, floa
   return hipSuccess;
 }
 
-template <int MUL>
-__device__ __forceinline__ float2 kmul(const ColArgs& a, int x, int r, float kxv, float kyv, int2 pr) {
+template <int MUL, class A>
+__device__ __forceinline__ float2 kmul(const A& a, int x, int r, float kxv, float kyv, int2 pr) {
   if constexpr (MUL == CM_SCALE) {
     return make_float2(a.scale, 0.f);
   } else if constexpr (MUL == CM_PROP || MUL == CM_PROPCONJ) {
-    // inside |k| < kmax: rows tabulated per column by k_prop_rows (no
-    // frequency loads or square roots here, and no branch around the load)
-    float2 p = cmul(a.px[x], a.py[r]);
-    if (MUL == CM_PROPCONJ) p.y = -p.y;
-    return (r <= pr.x || r >= pr.y) ? cscale(p, a.scale) : make_float2(0.f, 0.f);
+    // not reached: col_body applies the propagator itself
+    return make_float2(0.f, 0.f);
   } else if constexpr (MUL == CM_BAND) {
     // BandLimitKernel, U.cu:1037-1040: keeps |k| <= kmax
     const float k = sqrtf(kxv * kxv + kyv * kyv);
     return make_float2(k <= a.kmax ? a.scale : 0.f, 0.f);
+  } else if constexpr (MUL == CM_BANDGRAD) {
+    // BandLimitKernel then gradKernel (U.cu:1037-1040, 959-960) as one factor
+    const float k = sqrtf(kxv * kxv + kyv * kyv);
+    const float gr = -4.f * 3.14159f * 3.14159f * (kxv * kxv + kyv * kyv);
+    return make_float2(k <= a.kmax ? a.scale * gr : 0.f, 0.f);
   } else {  // CM_GRAD: gradKernel, U.cu:959-960
     return make_float2(a.scale * -4.f * 3.14159f * 3.14159f * (kxv * kxv + kyv * kyv), 0.f);
   }
@@ -367,21 +424,35 @@ template <int MODE> struct ColModeTraits {
 };
 
 // FFT / multiply / IFFT of one column held canonically by T threads
-template <class P, int N, int MODE, int MUL, class S, class LL>
-__device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, const ColArgs& a, const LL& L,
+template <class P, int N, int MODE, int MUL, class S, class LL, class A>
+__device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, const A& a, const LL& L,
                                          const float2* tw) {
   using Tr = ColModeTraits<MODE>;
   constexpr int T = P::T, E = N / T;
   if constexpr (Tr::FWD) fft_line<P, N, -1, S>(v, t, tw, L);
   // frequencies are read only by modes that need them (scale-only passes may
   // run without k-grids)
-  constexpr bool NEEDK = (Tr::MULT && (MUL == CM_BAND || MUL == CM_GRAD));
+  constexpr bool NEEDK = (Tr::MULT && (MUL == CM_BAND || MUL == CM_GRAD || MUL == CM_BANDGRAD));
   constexpr bool NEEDROWS = (Tr::MULT && (MUL == CM_PROP || MUL == CM_PROPCONJ));
   float kxv = 0.f;
   if constexpr (NEEDK) kxv = a.kx[x];
   int2 pr = make_int2(0, 0);
   if constexpr (NEEDROWS) pr = a.prop_rows[x];
-  if constexpr (Tr::MULT) {
+  if constexpr (NEEDROWS) {
+    // propagator inside |k| < kmax (rows per column from k_prop_rows): the
+    // factor is loaded for every row and zeroed by a multiplier, so the
+    // compiler issues the loads together instead of one branch (and one
+    // memory round trip) per element
+    const float2 pxv = a.px[x];
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const int r = t + c * T;
+      float2 p = cmul(pxv, a.py[r]);
+      if constexpr (MUL == CM_PROPCONJ) p.y = -p.y;
+      const float m = (r <= pr.x || r >= pr.y) ? a.scale : 0.f;
+      v[c] = cmul(v[c], cscale(p, m));
+    }
+  } else if constexpr (Tr::MULT) {
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const int r = t + c * T;
@@ -400,8 +471,8 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, co
 // (col, t) holds rows t + c T of column col; workgroup barriers
 // launched tile bx (of nbx per wave) of wave b; lds holds Cfg::LDS elements,
 // stw the twiddles of length N
-template <int N, int MODE, int MUL, class Cfg>
-__device__ __forceinline__ void col_block(const ColArgs& a, int bx, int b, int nbx, float2* lds, const float2* stw) {
+template <int N, int MODE, int MUL, class Cfg, class A>
+__device__ __forceinline__ void col_block(const A& a, int bx, int b, int nbx, float2* lds, const float2* stw) {
   constexpr int T = Cfg::T, E = N / T, C = Cfg::C;
   const int col = threadIdx.x % C, t = threadIdx.x / C;
   // band-limited runs launch only the tiles at both ends of kx
@@ -581,7 +652,7 @@ __global__ void __launch_bounds__((ColStream<N>::THREADS)) k_col_stream(ColArgs 
 // per-size passes use where the shape allows.
 template <int N> struct GroupRow { static constexpr int RV = N == 512 ? 2 : 1; };
 template <int N> struct GroupCol {
-  using P = typename std::conditional<N == 256, AltPlan<N>, Plan<N>>::type;
+  using P = Plan<N>;
   static constexpr bool INTERLEAVED = true, DUAL = false;
   static constexpr int T = P::T, THREADS = 512, C = THREADS / T, SH = 0, LSC = 0, LDS = N * C;
   static_assert(C * T == THREADS && C <= N / 4, "tile does not fit the band");
@@ -603,7 +674,7 @@ __device__ __forceinline__ int group_seg(const PassGroup<A>* g, long long it, fl
   const int nseg = g->nseg;
   int s = 0;
   while (s + 1 < nseg && it >= g->blk0[s + 1]) ++s;
-  const float2* tw = g->a[s].tw;
+  const float2* tw = reinterpret_cast<const typename GlobalView<A>::T*>(&g->a[s])->tw;
   const int n = g->a[s].nx;
   for (int i = threadIdx.x; i < n; i += nthreads) stw[i] = tw[i];
   __syncthreads();
@@ -614,16 +685,21 @@ template <int MODE, int MUL, int FL>
 __global__ void __launch_bounds__(256) k_row_group(const PassGroup<RowArgs>* g) {
   __shared__ float2 lds[kGroupRowLds];
   __shared__ float2 stw[kGroupTw];
-  const long long it = blockIdx.x;
+  // XCD-aware, and at 2048 wave-interleaved inside the segment (as k_row)
+  const long long it = xcd_remap(blockIdx.x, gridDim.x);
   if (it >= g->blk0[g->nseg]) return;
   const int s = group_seg(g, it, stw, 256);
-  const RowArgs& a = g->a[s];
+  const RowArgsG& a = reinterpret_cast<const RowArgsG*>(g->a)[s];
   const long long blk = it - g->blk0[s];
   switch (a.nx) {
     case 256: row_block<256, MODE, MUL, FL, GroupRow<256>::RV>(a, blk, lds, stw); break;
     case 512: row_block<512, MODE, MUL, FL, GroupRow<512>::RV>(a, blk, lds, stw); break;
     case 1024: row_block<1024, MODE, MUL, FL, GroupRow<1024>::RV>(a, blk, lds, stw); break;
-    default: row_block<2048, MODE, MUL, FL, GroupRow<2048>::RV>(a, blk, lds, stw); break;
+    default: {
+      constexpr int ybs = 2048 / RowCfg<2048, GroupRow<2048>::RV>::LINES;
+      row_block<2048, MODE, MUL, FL, GroupRow<2048>::RV>(a, row_interleave(blk, a.nbatch, ybs), lds, stw);
+      break;
+    }
   }
 }
 
@@ -636,7 +712,7 @@ __global__ void __launch_bounds__(512) k_col_group(const PassGroup<ColArgs>* g) 
   const long long it = xcd_remap(blockIdx.x, gridDim.x);
   if (it >= g->blk0[g->nseg]) return;
   const int s = group_seg(g, it, stw, 512);
-  const ColArgs& a = g->a[s];
+  const ColArgsG& a = reinterpret_cast<const ColArgsG*>(g->a)[s];
   const long long blk = it - g->blk0[s];
   const int nbx = g->nbx[s], b = (int)(blk / nbx), bx = (int)(blk - (long long)b * nbx);
   switch (a.nx) {
@@ -744,6 +820,7 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   dim3 grid((unsigned)((nlines + LINES - 1) / LINES)), block(LINES * T);
   const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0) | (a.aux ? STORE_AUX : 0);
   if (a.aux && mode != ROW_INV_MUL_FWD) return hipErrorInvalidValue;
+  if (mode == ROW_INV_FD_FWD && (!a.fd_pm || !a.fd_p0 || !a.t)) return hipErrorInvalidValue;
 #define MSL_ROW(M, U, F) hipLaunchKernelGGL((k_row<N, M, U, F, RV>), grid, block, 0, s, a)
   if (mode == ROW_FWD && mul == RM_NONE && fl == 0) MSL_ROW(ROW_FWD, RM_NONE, 0);
   else if (mode == ROW_FWD && mul == RM_NONE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_NONE, PRUNE_OUT);
@@ -763,6 +840,9 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == STORE_AUX) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, STORE_AUX);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == (PRUNE_IN | PRUNE_OUT | STORE_AUX))
     MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, PRUNE_IN | PRUNE_OUT | STORE_AUX);
+  else if (mode == ROW_INV_FD_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT))
+    MSL_ROW(ROW_INV_FD_FWD, RM_T, PRUNE_IN | PRUNE_OUT);
+  else if (mode == ROW_INV_FD_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_INV_FD_FWD, RM_T, 0);
   else return hipErrorInvalidValue;
 #undef MSL_ROW
   return hipGetLastError();
@@ -797,6 +877,7 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
     else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COL(COL_FWD_MUL_INV, CM_PROPCONJ);
     else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COL(COL_FWD_MUL_INV, CM_BAND);
     else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COL(COL_FWD_MUL_INV, CM_GRAD);
+    else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COL(COL_FWD_MUL_INV, CM_BANDGRAD);
     else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COL(COL_FWD_MUL_INV, CM_SCALE);
     else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COL(COL_FWD_MUL, CM_PROP);
     else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COL(COL_FWD_MUL, CM_SCALE);
@@ -850,6 +931,7 @@ static hipError_t col_stream(int mode, int mul, const ColArgs& a, hipStream_t s)
   else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ);
   else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLS(COL_FWD_MUL_INV, CM_BAND);
   else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLS(COL_FWD_MUL_INV, CM_GRAD);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COLS(COL_FWD_MUL_INV, CM_BANDGRAD);
   else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL_INV, CM_SCALE);
   else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COLS(COL_FWD_MUL, CM_PROP);
   else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL, CM_SCALE);

@@ -258,8 +258,16 @@ constexpr int ACAP = 1024;  // atoms staged in LDS per chunk
 // owns a 16x16 pixel tile of one slice; the atoms of every CSR bin within
 // sqrt(5) A of the tile (slab window per mode) are staged through LDS in
 // chunks, then each thread sums its pixel's contributions.
+// the potential arguments as device code reads them from a table in memory
+// (pointers in the global address space, same layout)
+struct PotArgsG {
+  MSL_POT_ARGS(MSL_GLOBAL)
+};
+static_assert(sizeof(PotArgsG) == sizeof(PotArgs), "layouts differ");
+
 // pixel tile (tx, ty) of slice `slice` (relative to first_slice); 256 threads
-__device__ __forceinline__ void potential_tile(const PotArgs& a, int tx, int ty, int slice) {
+template <class A>
+__device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int slice) {
   __shared__ float zt[103 * 24];
   __shared__ float sx[ACAP], sy[ACAP], sz[ACAP];
   __shared__ int sZ[ACAP];
@@ -357,11 +365,11 @@ __device__ __forceinline__ void potential_tile(const PotArgs& a, int tx, int ty,
   if (a.out_kind == 0) {
     float sn, cs;
     sincosf(a.sigma * sum, &sn, &cs);
-    reinterpret_cast<float2*>(a.out)[o] = make_float2(cs, sn);
+    ((float2*)a.out)[o] = make_float2(cs, sn);
   } else if (a.out_kind == 1) {
-    reinterpret_cast<float2*>(a.out)[o] = make_float2(sum, 0.f);
+    ((float2*)a.out)[o] = make_float2(sum, 0.f);
   } else {
-    reinterpret_cast<float*>(a.out)[o] = sum;
+    ((float*)a.out)[o] = sum;
   }
 }
 
@@ -378,7 +386,7 @@ __global__ void __launch_bounds__(256) k_potential_group(const PotArgs* tab, con
     if (blk0[mid] <= id) lo = mid;
     else hi = mid - 1;
   }
-  PotArgs a = tab[lo];
+  PotArgsG a = reinterpret_cast<const PotArgsG*>(tab)[lo];
   a.first_slice = first;
   const int tx = nblk(a.nx, PT);
   const long long local = id - blk0[lo];

@@ -247,4 +247,12 @@ template <int N, int DIR> __device__ __forceinline__ void fft_line(float2 (&v)[N
   fft_line<Plan<N>, N, DIR, BlockSync, LdsLine<0>>(v, t, tw, lds);
 }
 
+// pointer qualifier of the global address space (see MSL_ROW_ARGS); the
+// host pass of a .hip file sees plain pointers (same layout)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MSL_GLOBAL __attribute__((address_space(1)))
+#else
+#define MSL_GLOBAL
+#endif
+
 }  // namespace msl

@@ -9,60 +9,79 @@ namespace msl {
 
 // ------------------------------------------------------------------ passes
 // Row pass: one transform along x (contiguous) per (wave, row) line.
-enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_INV_MUL_FWD = 2 };
+enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_INV_MUL_FWD = 2, ROW_INV_FD_FWD = 3 };
+// ROW_INV_FD_FWD: the finite-difference step of one row (finiteDifferenceKernel,
+// U.cu:995-1023) between the transforms: IFFT_x of the Laplacian spectrum
+// (src), psi+ from it, psi- (fd_pm), psi (fd_p0) and V (t, real part, slot
+// indexing as RM_T), then FFT_x of psi+ into dst.
 // pointwise factor applied between the (optional) inverse and forward
 // transforms: none, a stored transmission t, or exp(i sigma V) from a real
 // potential
 enum RowMul { RM_NONE = 0, RM_T = 1, RM_PHASE = 2 };
 
+// The launch structures are written once as field lists parameterised by a
+// pointer qualifier G: empty for the host-visible structs below, the global
+// address space for the copies device code reads from tables in memory
+// (a pointer loaded from memory is otherwise generic to the compiler, and
+// every access through it a flat instruction).
+#define MSL_ROW_ARGS(G)                                                                                   \
+  const float2 G* src;                                                                                   \
+  float2 G* dst;                                                                                         \
+  long long src_bstride, dst_bstride; /* elements between consecutive waves */                           \
+  int nx, ny, nbatch;                                                                                    \
+  /* RM_T: transmission of wave b at t + (b / t_group) * t_bstride, or at t + t_index[b] * t_bstride */ \
+  const float2 G* t;                                                                                     \
+  const float G* V; /* RM_PHASE: potential, same indexing */                                             \
+  long long t_bstride;                                                                                   \
+  int t_group;                                                                                           \
+  float sigma;                                                                                           \
+  float scale;               /* applied with the pointwise factor */                                     \
+  int prune_in, prune_out;   /* band-limited input / output (only x < N/4, x >= 3N/4) */                 \
+  const float2 G* tw;        /* twiddles for length nx */                                                \
+  const int G* t_index;      /* optional per-wave transmission slot (device) */                          \
+  /* ROW_INV_MUL_FWD: also store the real-space wave (before the multiply) here, same indexing as dst */ \
+  float2 G* aux;                                                                                         \
+  /* ROW_INV_FD_FWD: psi- and psi in real space (dst indexing), slice step and wavelength */             \
+  const float2 G* fd_pm;                                                                                 \
+  const float2 G* fd_p0;                                                                                 \
+  float fd_dz, fd_wavel;
 struct RowArgs {
-  const float2* src;
-  float2* dst;
-  long long src_bstride, dst_bstride;  // elements between consecutive waves
-  int nx, ny, nbatch;
-  const float2* t;        // RM_T: transmission of wave b at t + (b / t_group) * t_bstride,
-                          // or at t + t_index[b] * t_bstride when t_index is set
-  const float* V;         // RM_PHASE: potential, same indexing
-  long long t_bstride;
-  int t_group;
-  float sigma;
-  float scale;            // applied with the pointwise factor
-  int prune_in, prune_out; // band-limited input / output (only x < N/4, x >= 3N/4)
-  const float2* tw;       // twiddles for length nx
-  const int* t_index;     // optional per-wave transmission slot (device)
-  float2* aux;            // ROW_INV_MUL_FWD: also store the real-space wave (before the
-                          // multiply) here, same indexing as dst (null: not stored)
+  MSL_ROW_ARGS()
 };
 
 // Column pass: one transform along y (stride nx) per (wave, column).
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FWD_MUL_INV = 2, COL_FWD_MUL = 3, COL_FWD_DETECT = 4, COL_MUL_INV = 5 };
 // k-space factor: none (scale only), Fresnel propagator with strict
 // |k| < kmax, band limit |k| <= kmax, spectral Laplacian, conjugate propagator
-enum ColMul { CM_SCALE = 0, CM_PROP = 1, CM_BAND = 2, CM_GRAD = 3, CM_PROPCONJ = 4 };
+enum ColMul { CM_SCALE = 0, CM_PROP = 1, CM_BAND = 2, CM_GRAD = 3, CM_PROPCONJ = 4, CM_BANDGRAD = 5 };
+// CM_BANDGRAD: band limit and spectral Laplacian in one factor (the Laplacian
+// of a band-limited wave, from the spectrum before the band limit)
 
-struct ColArgs {
-  const float2* src;
-  float2* dst;
-  long long src_bstride, dst_bstride;
-  int nx, ny, nbatch;
-  const float* kx;        // FP32 spatial frequencies (1/A)
-  const float* ky;
-  const float2* px;       // separable propagator factors exp(-i pi lambda dz k^2)
-  const float2* py;
-  const int2* prop_rows;  // CM_PROP / CM_PROPCONJ: column x is inside |k| < kmax at rows
-                          // r <= prop_rows[x].x and r >= prop_rows[x].y (launch_prop_rows)
-  float kmax;
-  float scale;
-  const float2* tw;       // twiddles for length ny
-  // COL_FWD_DETECT: annular sum of |v|^2 over kin <= |k| < kout, times
-  // det_scale, into partial[b * (nx / C) + tile]; optional per-wave
-  // |v|^2 map (fftshifted) into inten
-  float kin, kout, det_scale;
-  float* partial;
-  float* inten;
-  // > 0: launch only the tiles_lo column tiles at each end of kx (band-limited
-  // data); COL_FWD_DETECT partials are then indexed by launched tile
+#define MSL_COL_ARGS(G)                                                                                   \
+  const float2 G* src;                                                                                   \
+  float2 G* dst;                                                                                         \
+  long long src_bstride, dst_bstride;                                                                    \
+  int nx, ny, nbatch;                                                                                    \
+  const float G* kx;         /* FP32 spatial frequencies (1/A) */                                        \
+  const float G* ky;                                                                                     \
+  const float2 G* px;        /* separable propagator factors exp(-i pi lambda dz k^2) */                 \
+  const float2 G* py;                                                                                    \
+  /* CM_PROP / CM_PROPCONJ: column x is inside |k| < kmax at rows r <= prop_rows[x].x and */            \
+  /* r >= prop_rows[x].y (make_prop_rows) */                                                             \
+  const int2 G* prop_rows;                                                                               \
+  float kmax;                                                                                            \
+  float scale;                                                                                           \
+  const float2 G* tw;        /* twiddles for length ny */                                                \
+  /* COL_FWD_DETECT: annular sum of |v|^2 over kin <= |k| < kout, times det_scale, into */              \
+  /* partial[b * (nx / C) + tile]; optional per-wave |v|^2 map (fftshifted) into inten */                \
+  float kin, kout, det_scale;                                                                            \
+  float G* partial;                                                                                      \
+  float G* inten;                                                                                        \
+  /* > 0: launch only the tiles_lo column tiles at each end of kx (band-limited data); */                \
+  /* COL_FWD_DETECT partials are then indexed by launched tile */                                        \
   int tiles_lo;
+struct ColArgs {
+  MSL_COL_ARGS()
 };
 
 // rows of each column inside the propagator band |k| < kmax (nx entries,
@@ -131,24 +150,29 @@ hipError_t launch_bin(const BinArgs& a, hipStream_t s);
 // 2: point 3D potential on the plane (finite-difference mode, no offset).
 // out_kind 0: transmission exp(i sigma V) as float2, 1: real V as float2
 // (V, 0), 2: real V as float.
+#define MSL_POT_ARGS(G)                                                                                   \
+  const float G* bx;                                                                                     \
+  const float G* by;                                                                                     \
+  const float G* bz;                                                                                     \
+  const int G* bZ;                                                                                       \
+  const int G* start;                                                                                    \
+  int nslab;                                                                                             \
+  const float G* ztab;       /* per-Z transformed Kirkland constants (103 x 12) */                       \
+  float MaxX, MinX, MaxY, MinY, MaxZ, MinZ;                                                              \
+  int nx, ny;                                                                                            \
+  float ps, offx, offy;                                                                                  \
+  float dz;                  /* slice thickness (modes 0, 1) */                                          \
+  int first_slice, nslices;  /* slice i uses z = (MaxZ-MinZ) - dz * i (modes 0, 1) */                    \
+  float zplane;              /* modes 2, 3: plane position (single slice) */                             \
+  int loadslicesz;                                                                                       \
+  float sigma;                                                                                           \
+  /* mode 0 3D-integrated slice, 1 projected, 2 FD plane, 3 real-space 1 A slab; */                     \
+  /* out_kind 0 exp(i sigma V), 1 (V, 0), 2 V */                                                         \
+  int mode, out_kind;                                                                                    \
+  void G* out;               /* [nslices][ny][nx] */                                                     \
+  long long out_sstride;     /* elements between slices */
 struct PotArgs {
-  const float *bx, *by, *bz;
-  const int* bZ;
-  const int* start;
-  int nslab;
-  const float* ztab;            // per-Z transformed Kirkland constants (103 x 12)
-  float MaxX, MinX, MaxY, MinY, MaxZ, MinZ;
-  int nx, ny;
-  float ps, offx, offy;
-  float dz;                     // slice thickness (modes 0, 1)
-  int first_slice, nslices;     // slice i uses z = (MaxZ-MinZ) - dz * i (modes 0, 1)
-  float zplane;                 // modes 2, 3: plane position (single slice)
-  int loadslicesz;
-  float sigma;
-  int mode, out_kind;           // mode 0 3D-integrated slice, 1 projected, 2 FD plane,
-                                // 3 real-space 1 A slab; out_kind 0 exp(i sigma V), 1 (V, 0), 2 V
-  void* out;                    // [nslices][ny][nx]
-  long long out_sstride;        // elements between slices
+  MSL_POT_ARGS()
 };
 hipError_t launch_potential(const PotArgs& a, hipStream_t s);
 // one slice (first_slice = first, nslices 1) of several jobs in one launch:

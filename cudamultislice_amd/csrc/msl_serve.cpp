@@ -84,6 +84,9 @@ class Runner {
     np_ = (long long)n_ * n_;
     ntiles_ = n_ / col_tile(n_);
     prune_ = !fd && cs->prune_tiles > 0;
+    fd_prune_ = fd && cs->prune_tiles > 0;
+    for (int x = n_ / 4; x < 3 * n_ / 4 && fd_prune_; ++x)
+      if (std::sqrt(cs->hkx[x] * cs->hkx[x] + 0.0f) <= cs->kmax) fd_prune_ = false;
   }
 
   // jobs of one class, sharing the tables of cs
@@ -95,7 +98,7 @@ class Runner {
     if (probes) probe_norm();
     else scratch_.ensure(np_);
     const long long budget = p_.budget_bytes > 0 ? p_.budget_bytes : (32LL << 30);
-    const long long wave_b = np_ * 8 * (fd ? 5 : 1), slot_b = np_ * 8 * (fd ? 2 : 1);
+    const long long wave_b = np_ * 8 * (fd ? (fd_prune_ ? 6 : 5) : 1), slot_b = np_ * 8 * (fd ? 2 : 1);
     const long long maxw = std::max<long long>(1, (budget - slot_b) / (wave_b + slot_b));
     std::vector<Item> items;
     for (int j : ids) {
@@ -295,6 +298,8 @@ class Runner {
   Stats& st_;
   msl_sim* cs_ = nullptr;
   bool fd_ = false, prune_ = false;
+  bool fd_prune_ = false;  // finite differences: band limit |k| <= kmax leaves N/4 <= x < 3N/4 empty
+  DBuf<float2> Z1_;
   int n_ = 0, ntiles_ = 0;
   long long np_ = 0;
   float norm_ = 1.f;
@@ -489,6 +494,7 @@ class Runner {
     DBuf<float2> Pm, P0, Pp, G, Bt, V, Vt;
     DBuf<float> partial, adf;
     Pm.ensure(wn); P0.ensure(wn); Pp.ensure(wn); G.ensure(wn); Bt.ensure(wn);
+    if (fd_prune_) Z1_.ensure(wn);
     V.ensure((size_t)J * np_); Vt.ensure((size_t)J * np_);
     partial.ensure((size_t)L.nw * ntiles_);
     adf.ensure(L.nw);
@@ -516,6 +522,60 @@ class Runner {
     xform(pm, p0, p0, L.nw, CM_PROP);
     const double t0 = now_ms();
     const float dzfd = cs_->dzFD;
+    // Band-limited grids: the step runs fused.  Y holds the Laplacian of psi
+    // in the (kx, y) domain.  Per step: one row pass (IF_x of Y, the
+    // finite-difference update, F_x of psi+ into Z), two column passes over the
+    // in-band columns of Z (band limit -> Z1, band limit times Laplacian -> Y
+    // for the next step) and one row pass IF_x(Z1) -> band-limited psi+: four
+    // passes instead of the three transforms and the update of U.cu:2040-2083.
+    if (fd_prune_) {
+      float2 *Y = G.p, *Z = Bt.p, *Z1 = Z1_.p;
+      row(cs_, ROW_FWD, RM_NONE, row_args(cs_, p0, Y, L.nw, np_));
+      {
+        ColArgs c = col_args(cs_, Y, Y, L.nw, np_);
+        c.scale = nf2;
+        col(cs_, COL_FWD_MUL_INV, CM_GRAD, c);
+      }
+      for (int it = 1; it <= b.front().nslices; ++it) {
+        const int A = active_waves(b, L, it), Ja = active_jobs(b, jobs_, it);
+        for (int j : L.jobs) {
+          Job& jb = jobs_[j];
+          if (jb.d->nslices < it) continue;
+          const float zc = (jb.s->MaxZ - jb.s->MinZ) - dzfd * (float)it;
+          potential(jb.s.get(), 2, 0, 1, 1.0f, zc, 1, V.p + (size_t)jb.slot * np_, np_);
+        }
+        xform(V.p, Vt.p, V.p, Ja, CM_BAND);  // band-limited V
+        RowArgs r = row_args(cs_, Y, Z, A, np_);
+        r.prune_in = r.prune_out = 1;
+        r.t = V.p;
+        r.t_index = L.tidx.p;
+        r.t_bstride = np_;
+        r.fd_pm = pm;
+        r.fd_p0 = p0;
+        r.fd_dz = dzfd;
+        r.fd_wavel = cs_->wavel;
+        r.sigma = cs_->sigma2;
+        row(cs_, ROW_INV_FD_FWD, RM_T, r);
+        ColArgs c = col_args(cs_, Z, Z1, A, np_);
+        c.scale = nf2;
+        c.tiles_lo = cs_->prune_tiles;
+        col(cs_, COL_FWD_MUL_INV, CM_BAND, c);
+        c.dst = Y;
+        col(cs_, COL_FWD_MUL_INV, CM_BANDGRAD, c);
+        RowArgs ri = row_args(cs_, Z1, pp, A, np_);
+        ri.prune_in = 1;
+        row(cs_, ROW_INV, RM_NONE, ri);
+        st_.launches += 4;
+        std::swap(pm, p0);
+        std::swap(p0, pp);
+        st_.wave_slices += A;
+        for (size_t k = 0; k < b.size(); ++k)
+          if (b[k].nslices == it) finish_fd(b, L, k, p0, Z, partial.p, adf.p);
+      }
+      HIPC(hipStreamSynchronize(st()));
+      st_.prop += now_ms() - t0;
+      return;
+    }
     for (int it = 1; it <= b.front().nslices; ++it) {
       const int A = active_waves(b, L, it), Ja = active_jobs(b, jobs_, it);
       for (int j : L.jobs) {
@@ -536,28 +596,33 @@ class Runner {
       std::swap(pm, p0);
       std::swap(p0, pp);
       st_.wave_slices += A;
-      for (size_t k = 0; k < b.size(); ++k) {
-        if (b[k].nslices != it) continue;
-        const Job& jb = jobs_[b[k].job];
-        float2* w = p0 + (size_t)L.w0[k] * np_;
-        if (jb.d->nprobe > 0) {
-          float2* f = Bt.p + (size_t)L.w0[k] * np_;
-          row(cs_, ROW_FWD, RM_NONE, row_args(cs_, w, f, b[k].cnt, np_));
-          ColArgs dc = col_args(cs_, f, nullptr, b[k].cnt, np_);
-          dc.kin = p_.adf_inner * 1e-3f / cs_->wavel;
-          dc.kout = p_.adf_outer * 1e-3f / cs_->wavel;
-          dc.det_scale = nf2;
-          dc.partial = partial.p + (size_t)L.w0[k] * ntiles_;
-          col(cs_, COL_FWD_DETECT, CM_SCALE, dc);
-          HIPC(launch_reduce_rows(dc.partial, b[k].cnt, ntiles_, adf.p + L.w0[k], 0, st()));
-          adf_out(b, k, adf.p + L.w0[k]);
-        } else if (jb.d->exit_out) {
-          HIPC(hipMemcpyAsync(jb.d->exit_out, w, np_ * sizeof(float2), hipMemcpyDeviceToHost, st()));
-        }
-      }
+      for (size_t k = 0; k < b.size(); ++k)
+        if (b[k].nslices == it) finish_fd(b, L, k, p0, Bt.p, partial.p, adf.p);
     }
     HIPC(hipStreamSynchronize(st()));
     st_.prop += now_ms() - t0;
+  }
+
+  // item k of a finite-difference batch after its last step: detector
+  // (probes, through scratch waves f) or exit wave; W in real space
+  void finish_fd(const std::vector<Item>& b, const Layout& L, size_t k, float2* W, float2* f, float* partial,
+                 float* adf) {
+    const Job& jb = jobs_[b[k].job];
+    float2* w = W + (size_t)L.w0[k] * np_;
+    if (jb.d->nprobe > 0) {
+      float2* fw = f + (size_t)L.w0[k] * np_;
+      row(cs_, ROW_FWD, RM_NONE, row_args(cs_, w, fw, b[k].cnt, np_));
+      ColArgs dc = col_args(cs_, fw, nullptr, b[k].cnt, np_);
+      dc.kin = p_.adf_inner * 1e-3f / cs_->wavel;
+      dc.kout = p_.adf_outer * 1e-3f / cs_->wavel;
+      dc.det_scale = cs_->nf * cs_->nf;
+      dc.partial = partial + (size_t)L.w0[k] * ntiles_;
+      col(cs_, COL_FWD_DETECT, CM_SCALE, dc);
+      HIPC(launch_reduce_rows(dc.partial, b[k].cnt, ntiles_, adf + L.w0[k], 0, st()));
+      adf_out(b, k, adf + L.w0[k]);
+    } else if (jb.d->exit_out) {
+      HIPC(hipMemcpyAsync(jb.d->exit_out, w, np_ * sizeof(float2), hipMemcpyDeviceToHost, st()));
+    }
   }
 };
 

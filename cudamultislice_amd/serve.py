@@ -46,10 +46,11 @@ def _p(a):
 
 def serve(jobs: list[Job], kV: float = 300.0, ps: float = 0.1, dz: float = 2.0, aperture: float = 25.0,
           adf=(60.0, 200.0), potential_mode: int = 0, budget_bytes: int = 0, device: int = 0,
-          group_sizes: bool = True) -> dict:
+          group_sizes: bool = False) -> dict:
     """Runs every job; fills job.result ("exit" for plane waves, "adf" for
     probes) and returns the engine's statistics.  group_sizes: conventional
-    jobs of grid 256..2048 share one launch per pass and slice."""
+    jobs of grid 256..2048 share one launch per pass and slice (half the
+    launches; on MI355X the per-size launches are faster for large jobs)."""
     L = lib()
     keep = []
     arr = (msl_job * len(jobs))()

@@ -166,6 +166,35 @@ def test_example_tem_program_full_si_case_matches_oracle(fp, tmp_path):
     assert rel_l2(img, np.abs(det)) < 1e-3
 
 
+def test_example_tem_program_under_address_sanitizer(tmp_path):
+    """The facade program with AddressSanitizer / LeakSanitizer on every host
+    object (kernels uninstrumented): no invalid access and no leaked host
+    allocation of ours across the whole GUI sequence (load, bin, physics,
+    frequencies, slices, exit wave, CTF + detector image, FreeExitWave,
+    destruction).  The ROCm runtime's own allocations held until exit are
+    suppressed (csrc/lsan.supp)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "cudamultislice_amd", "example_tem_asan")
+    if not os.path.exists(exe):
+        pytest.skip("example_tem_asan not built (make -C cudamultislice_amd/csrc)")
+    Z, x, y, z = S.silicon(2, 2, 4)
+    path = tmp_path / "si.xyz"
+    with open(path, "w") as f:
+        f.write(f"{Z.size}\n\n")
+        for a, b, c in zip(x, y, z):
+            f.write(f"Si {a:.6f} {b:.6f} {c:.6f}\n")
+    env = dict(os.environ)
+    env["ASAN_OPTIONS"] = "detect_leaks=1:exitcode=23"
+    env["LSAN_OPTIONS"] = "suppressions=" + os.path.join(root, "cudamultislice_amd", "csrc", "lsan.supp")
+    r = subprocess.run([exe, str(path), str(tmp_path / "out"), "128", "0.1", "200", "1", "1000"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "Sanitizer" not in r.stderr, r.stderr[-4000:]
+    assert r.stdout.startswith("slices ")
+
+
 def test_tem_si_1024_252_slices_full_size_properties(fp):
     """Si 14x14x92 cells (144 256 atoms), 1024^2 over 76.02 A, 2 A slices,
     300 kV: at full size the exit wave is finite, the band-limited
@@ -656,6 +685,55 @@ def test_diff_checkpoint_interval_does_not_change_gradient():
             assert np.array_equal(e.backward(gI).cpu().numpy(), grads[-1])
     # other intervals move the real-space waves through other passes (rounding only)
     assert rel_l2(grads[1], grads[0]) < 1e-6 and rel_l2(grads[2], grads[0]) < 1e-6
+
+
+def test_handles_release_their_device_memory(fp):
+    """Every handle frees what it allocated (the reference leaks its plans
+    and frequency arrays on each PreCalculateFrequencies): device memory in
+    use returns to its level after repeated create / run / destroy cycles of
+    the simulation, batched STEM, differentiable and serving paths."""
+    import torch
+    from cudamultislice_amd import MultisliceSimulation
+    from cudamultislice_amd.diff import DiffEngine, MultisliceIntensity
+    from cudamultislice_amd.serve import Job, serve
+    Z, x, y, z = S.srtio3(2, 2, 3)
+    px, py = np.array([3, 40], np.int32), np.array([7, 50], np.int32)
+
+    def cycle():
+        with MultisliceSimulation() as g:
+            g.set_atoms(Z, x, y, z)
+            g.GetParameterisation()
+            g.UploadBinnedAtoms()
+            g.ApplyMicroscopeParameters(300.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 20.0)
+            g.SetCalculationVariables(0.2, 0.0, 128, 128, 1, 1, 12.0, 0.0, 0.0)
+            g.PreCalculateFrequencies()
+            g.PreCalculateFrequencies()
+            g.InitialiseWavefunctions(2.0)
+            for i in range(1, 4):
+                g.MultisliceStepConv(i)
+            g.GetExitWave()
+            g.STEMSetCalculationVariables(0.2, 0.0, 64, 64, 1, 1, 12.0)
+            g.PreCalculateFrequencies()
+            g.stem_batched(px, py, 2.0, nconfig=1, adf=(10.0, 200.0))
+        eng = DiffEngine(64, 64, 0.25, 300.0, 2.0, 4, 2, checkpoint_every=2)
+        V = torch.rand(4, 64, 64, device="cuda", requires_grad=True)
+        psi = torch.ones(2, 64, 64, dtype=torch.complex64, device="cuda")
+        MultisliceIntensity.apply(V, psi, eng).sum().backward()
+        eng.close()
+        serve([Job(64, Z, x, y, z, nslices=3, posx=px, posy=py), Job(64, Z, x, y, z, nslices=2)],
+              kV=300.0, ps=0.2, dz=2.0, aperture=25.0, adf=(10.0, 200.0))
+        torch.cuda.synchronize()
+
+    def used():
+        torch.cuda.synchronize()
+        free, total = torch.cuda.mem_get_info()
+        return total - free - torch.cuda.memory_reserved()
+
+    cycle()  # first use: runtime code objects and pools
+    base = used()
+    for _ in range(3):
+        cycle()
+    assert used() - base < 8 << 20, (used() - base)
 
 
 def test_long_run_100_slices_within_1e4(fp):
