@@ -56,6 +56,8 @@ static int lg2(int n) {
 //         5 as 0 in persistent workgroups that stream the next tile into a
 //           second LDS buffer (direct global-to-LDS loads) while the current
 //           one is transformed, so the strided column loads overlap compute
+//         6 the tile staged through LDS, then one column per wave with
+//           wave-local exchanges (two workgroup barriers per tile)
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 1-4 GiB
@@ -71,7 +73,7 @@ void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 3 : 6; }
+int num_pass_variants(int kind) { return kind == 0 ? 3 : 7; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
@@ -536,6 +538,97 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_
   col_block<N, MODE, MUL, Cfg>(a, (int)(gid % gridDim.x), (int)(gid / gridDim.x), (int)gridDim.x, lds, stw);
 }
 
+// ------------------------------------------------------------------ wave-local columns
+// Variant 6: the tile of C columns is read row by row (coalesced) into LDS,
+// one column per LDS line ([col][row], padding 1/16); after one workgroup
+// barrier each wave transforms its own column(s) with wave-local exchanges in
+// that line (no workgroup barriers inside the transforms), writes the result
+// back to the line, and after a second barrier the tile is stored row by row.
+// Lengths 128..1024 (the plan's T <= 64 keeps a column inside one wave).
+template <int N> struct ColWL {
+  using P = Plan<N>;
+  static constexpr int T = P::T, E = N / T;
+  static constexpr int C = N >= 512 ? 8 : 16;  // columns per workgroup
+  static constexpr int THREADS = C * T, SH = 4;
+  static constexpr int LS = LdsLine<SH>::span(N) + 1;
+  static_assert(T <= 64 && 64 % T == 0, "a column must stay inside one wave");
+};
+
+template <int N, int MODE, int MUL>
+__global__ void __launch_bounds__(ColWL<N>::THREADS) k_col_wl(ColArgs a) {
+  using W = ColWL<N>;
+  using LL = LdsLine<W::SH>;
+  constexpr int T = W::T, E = W::E, C = W::C;
+  __shared__ float2 lds[C * W::LS];
+  __shared__ float2 stw[N];
+  for (int i = threadIdx.x; i < N; i += W::THREADS) stw[i] = a.tw[i];
+  const long long gid = xcd_remap((long long)blockIdx.y * gridDim.x + blockIdx.x, (long long)gridDim.x * gridDim.y);
+  const int bx = (int)(gid % gridDim.x), b = (int)(gid / gridDim.x);
+  const int tile = (a.tiles_lo > 0 && bx >= a.tiles_lo) ? bx + a.nx / C - 2 * a.tiles_lo : bx;
+  const int x0 = tile * C;
+  const long long nx = a.nx;
+  // coalesced load: thread (cc, r0) reads rows r0 + k T of column x0 + cc
+  {
+    const int cc = threadIdx.x % C, r0 = threadIdx.x / C;
+    const float2* src = a.src + (long long)b * a.src_bstride + x0 + cc;
+    float2 u[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) u[k] = src[(long long)(r0 + k * T) * nx];
+    const LL L{lds + cc * W::LS, 1};
+#pragma unroll
+    for (int k = 0; k < E; ++k) L.at(r0 + k * T) = u[k];
+  }
+  __syncthreads();
+  const int col = threadIdx.x / T, t = threadIdx.x % T;
+  const LL L{lds + col * W::LS, 1};
+  float2 v[E];
+#pragma unroll
+  for (int c = 0; c < E; ++c) v[c] = L.at(t + c * T);
+  WaveSync::sync();  // the wave's reads of its line are done before the exchanges rewrite it
+  col_body<typename W::P, N, MODE, MUL, WaveSync>(v, t, x0 + col, a, L, stw);
+  WaveSync::sync();
+#pragma unroll
+  for (int c = 0; c < E; ++c) L.at(t + c * T) = v[c];
+  __syncthreads();
+  {
+    const int cc = threadIdx.x % C, r0 = threadIdx.x / C;
+    const LL Ls{lds + cc * W::LS, 1};
+    float2* dst = a.dst + (long long)b * a.dst_bstride + x0 + cc;
+#pragma unroll
+    for (int k = 0; k < E; ++k) dst[(long long)(r0 + k * T) * nx] = Ls.at(r0 + k * T);
+  }
+}
+
+template <int N>
+static hipError_t col_wl(int mode, int mul, const ColArgs& a, hipStream_t s) {
+  using W = ColWL<N>;
+  constexpr int C = W::C;
+  if (a.nbatch <= 0) return hipSuccess;
+  if (a.nx % C != 0 || a.ny != N) return hipErrorInvalidValue;
+  if (a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / ColTile0<N>::C) return hipErrorInvalidValue;
+  ColArgs b = a;
+  if (a.tiles_lo > 0) {
+    if ((a.nx / 4) % C != 0) return hipErrorInvalidValue;
+    b.tiles_lo = a.nx / 4 / C;
+  }
+  const dim3 grid(b.tiles_lo > 0 ? 2 * b.tiles_lo : a.nx / C, a.nbatch), block(W::THREADS);
+#define MSL_COLW(M, U) hipLaunchKernelGGL((k_col_wl<N, M, U>), grid, block, 0, s, b)
+  if (mode == COL_FWD) MSL_COLW(COL_FWD, CM_SCALE);
+  else if (mode == COL_INV) MSL_COLW(COL_INV, CM_SCALE);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLW(COL_FWD_MUL_INV, CM_PROP);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLW(COL_FWD_MUL_INV, CM_PROPCONJ);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLW(COL_FWD_MUL_INV, CM_BAND);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLW(COL_FWD_MUL_INV, CM_GRAD);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COLW(COL_FWD_MUL_INV, CM_BANDGRAD);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COLW(COL_FWD_MUL_INV, CM_SCALE);
+  else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COLW(COL_FWD_MUL, CM_PROP);
+  else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COLW(COL_FWD_MUL, CM_SCALE);
+  else if (mode == COL_MUL_INV && mul == CM_SCALE) MSL_COLW(COL_MUL_INV, CM_SCALE);
+  else return hipErrorInvalidValue;
+#undef MSL_COLW
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ streamed columns
 // Variant 5: a persistent workgroup walks the (wave, tile) items of the
 // launch; the [row][col] tile of the next item is fetched straight into the
@@ -947,6 +1040,10 @@ static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int 
   if (mode == COL_FWD_DETECT) {
     if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
     return var == 2 ? col_nv<N, 2>(mode, mul, a, s) : col_nv<N, 0>(mode, mul, a, s);
+  }
+  if (var == 6) {
+    if constexpr (N >= 128 && N <= 1024) return col_wl<N>(mode, mul, a, s);
+    else return col_nv<N, 0>(mode, mul, a, s);
   }
   if (var == 5) return col_stream<N>(mode, mul, a, s);
   if (var == 4) return col_nv<N, 4>(mode, mul, a, s);

@@ -158,13 +158,13 @@ msl_status msl_bench_pass(int kind, int variant, int n, int nbatch, int reps, fl
 
 // Autotuner: for length n and a batch of nbatch waves, time every row and
 // column variant that agrees with variant 0 to 1e-5 and select the fastest.
-// out (optional): [0..5] row ms per variant, [6..11] column ms per variant,
+// out (optional): [0..7] row ms per variant, [8..15] column ms per variant,
 // -1 for variants rejected or not built.
 msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* col_var, float* out) {
   if (!fft_size_supported(n) || nbatch < 1 || reps < 1) return MSL_ERR_ARG;
   int best[2] = {0, 0};
   if (out)
-    for (int i = 0; i < 12; ++i) out[i] = -1.f;
+    for (int i = 0; i < 16; ++i) out[i] = -1.f;
   for (int kind = 0; kind < 2; ++kind) {
     float bms = 1e30f;
     for (int v = 0; v < num_pass_variants(kind); ++v) {
@@ -172,7 +172,7 @@ msl_status msl_autotune_passes(int n, int nbatch, int reps, int* row_var, int* c
       msl_status st = msl_bench_pass(kind, v, n, nbatch, reps, &ms, &md);
       if (st != MSL_OK) return st;
       if (!(md < 1e-5f)) ms = -1;
-      if (out && v < 6) out[6 * kind + v] = ms;
+      if (out && v < 8) out[8 * kind + v] = ms;
       if (ms > 0 && ms < bms) {
         bms = ms;
         best[kind] = v;

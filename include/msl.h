@@ -224,7 +224,7 @@ msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, flo
  * k_pass.hip.  msl_bench_pass times `reps` launches on an n x n batch of
  * nbatch waves and, with maxdiff, compares against variant 0;
  * msl_autotune_passes selects the fastest agreeing variant per pass for n
- * (out[12]: [0..5] row and [6..11] column timings in ms per variant, -1 where
+ * (out[16]: [0..7] row and [8..15] column timings in ms per variant, -1 where
  * rejected or absent).  Timings use band-pruned passes where the length allows. */
 msl_status msl_set_pass_variant(int kind, int n, int variant);
 msl_status msl_get_pass_variant(int kind, int n, int* variant);
