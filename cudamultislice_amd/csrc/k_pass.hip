@@ -90,8 +90,11 @@ template <int N, int CV> struct ColCfg {
   static constexpr bool DUAL = CV == 4;
   static constexpr int LDS = INTERLEAVED ? N * C * (DUAL ? 2 : 1) : C * LSC;
   static constexpr int THREADS = C * T;
-  // no occupancy floor: forcing 4 waves per SIMD spills the long transforms
-  static constexpr int MIN_WAVES = 1;
+  // occupancy floor only for half-width tiles of short columns (E <= 8): six
+  // waves per SIMD (80 VGPRs, three workgroups per CU instead of two) ran 5 %
+  // faster at 512; the long transforms (E = 16) spill under any floor, and
+  // the full-width tiles ran slower with it
+  static constexpr int MIN_WAVES = (CV == 3 && N / T <= 8) ? 6 : 1;
 };
 
 int col_tile(int ny) {
