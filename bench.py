@@ -289,7 +289,8 @@ def main():
     ap.add_argument("--steps", type=int, default=16, help="frozen-phonon configurations timed")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--probes", type=int, default=4096)
-    ap.add_argument("--batch", type=int, default=0, help="probes per launch (0: all)")
+    ap.add_argument("--batch", type=int, default=0, help="probes per launch (0: the batch-width autotuner's choice)")
+    ap.add_argument("--no-tune-batch", action="store_true", help="with --batch 0: all probes in one launch")
     ap.add_argument("--per-probe", type=int, default=8, help="probes timed on the per-probe path (0: skip)")
     ap.add_argument("--cpu-probes", type=int, default=64)
     ap.add_argument("--cpu-slices", type=int, default=40)
@@ -340,6 +341,13 @@ def main():
     nprobe = px.size
     log(f"[bench] {atoms[0].size} atoms, {nslices} slices, {nprobe} probes, 512^2")
 
+    # batch width: the autotuner's choice (one chunk per candidate width over
+    # 8 slices, before the timed region) unless --batch fixes it
+    tuned = None
+    if args.batch == 0 and not args.no_tune_batch:
+        best, cost = sim.autotune_batch(px, py, dz, nslices=8, adf=(60.0, 200.0))
+        tuned = {"best": best, "us_per_probe_slice": {str(k): round(v, 4) for k, v in cost.items()}}
+        log(f"[bench] batch width autotuner: {tuned}")
     # warm-up configurations use seeds distinct from the timed ones
     sim.stem_batched(px, py, dz, nconfig=args.warmup, seed=S.SEED + 1000, sigma=0.08, batch=args.batch)
     torch.cuda.synchronize()
@@ -363,7 +371,7 @@ def main():
     # only the kx columns inside the propagator band (half) move; the row
     # pass also reads the shared transmission slice once.
     npix = 512 * 512
-    B = nprobe if args.batch <= 0 else min(args.batch, nprobe)
+    B = (tuned["best"] if tuned else nprobe) if args.batch <= 0 else min(args.batch, nprobe)
     kern = max(("row", "column"), key=lambda k: ks[k]["ms_total"])
     band = 0.5 if inf["band_tiles"] > 0 else 1.0
     bytes_launch = int(16 * B * npix * band) + (8 * npix if kern == "row" else 0)
@@ -379,7 +387,7 @@ def main():
     roof = {"bound": "hbm", "kernel": f"{kern} pass (512, batch {B})", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": "profiles/traffic_stem512.json (rocprofv3 FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, "
-                              "per wave of a 512-wave launch)" if traffic else None,
+                              "per wave, measured on launches of the same width)" if traffic else None,
             "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": ks[kern]["ms_avg"],
             "other_passes": {k: ks[k]["ms_avg"] for k in ("row", "column", "detector", "potential")}}
 
@@ -440,6 +448,7 @@ def main():
         "data": "synthetic (SrTiO3 supercell, seeded Philox frozen-phonon displacements)",
         "config": {"workload": "batched STEM + TDS, SrTiO3 10x10x50, 512^2 over 40 A, 300 kV, 2 A slices",
                    "probes": nprobe, "slices": nslices, "configs_per_step": 1, "batch": B,
+                   "batch_autotuner": tuned,
                    "adf_mrad": [60, 200], "step": "one frozen-phonon configuration"},
         "stage_ms_total": tm,
         "per_probe_path": per_probe,

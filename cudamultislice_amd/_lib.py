@@ -90,6 +90,7 @@ _SIGS = {
     "msl_set_pass_variant": ([_i, _i, _i], _i), "msl_get_pass_variant": ([_i, _i, C.POINTER(_i)], _i),
     "msl_bench_pass": ([_i, _i, _i, _i, _i, C.POINTER(_f), C.POINTER(_f)], _i),
     "msl_autotune_passes": ([_i, _i, _i, C.POINTER(_i), C.POINTER(_i), _vp], _i),
+    "msl_autotune_batch": ([_vp, C.POINTER(msl_stem_batch_desc), _i, C.POINTER(_i), _vp, _vp, C.POINTER(_i)], _i),
     "msl_copy_device": ([_vp, _vp, C.c_longlong, _vp], _i),
     "msl_get_binned": ([_vp, _vp, _vp, _vp, _vp, _vp], _i),
     "msl_get_bins": ([_vp, _vp], _i),

@@ -1136,6 +1136,60 @@ msl_status msl_kernel_stats(const msl_sim* s, double* out8) {
   return MSL_OK;
 }
 
+// Batch width: times the propagation of one chunk of B probes over a few
+// slices (probe synthesis, row / column passes, fused detector) for B = 64,
+// 128, ... up to the probes given (and ~32 GiB of waves), and keeps the width
+// with the least time per probe-slice for batch = 0 runs on this grid.
+msl_status msl_autotune_batch(msl_sim* s, const msl_stem_batch_desc* d, int nslices, int* best, int* widths,
+                              float* us_per_probe_slice, int* ncand) {
+  API_BEGIN(s)
+  require(d != nullptr && d->nprobe > 0 && d->posx && d->posy, MSL_ERR_ARG, "bad batch descriptor");
+  require(nslices >= 1, MSL_ERR_ARG, "nslices must be positive");
+  require(s->have_freqs, MSL_ERR_STATE, "STEMSetCalculationVariables and PreCalculateFrequencies first");
+  const long long np = (long long)s->resX * s->resY;
+  const int cap = (int)std::min<long long>(std::min(d->nprobe, 65535), (32LL << 30) / (np * 8));
+  std::vector<int> cand;
+  for (int b = 64; b < cap; b *= 2) cand.push_back(b);
+  cand.push_back(std::max(1, cap));
+  std::vector<float> adf((size_t)cap), cost(cand.size());
+  const int saved = s->tuned_batch;
+  s->tuned_batch = 0;
+  int pick = 0;
+  for (size_t k = 0; k < cand.size(); ++k) {
+    msl_stem_batch_desc t = *d;
+    t.nprobe = cand[k];
+    t.nconfig = 1;
+    t.sigma = 0.f;
+    t.nslices = nslices;
+    t.batch = cand[k];
+    t.adf_out = adf.data();
+    t.pacbed_out = nullptr;
+    t.exit_out = nullptr;
+    // the first run of a width also builds its graph and warms the caches
+    for (int rep = 0; rep < 2; ++rep) {
+      const msl_status st = msl_stem_batched(s, &t);
+      if (st != MSL_OK) {
+        s->tuned_batch = saved;
+        return st;
+      }
+    }
+    cost[k] = s->timings[2] * 1000.f / ((float)cand[k] * (float)nslices);
+    if (cost[k] < cost[pick]) pick = (int)k;
+  }
+  s->tuned_batch = cand[pick];
+  s->tuned_batch_n = s->resX;
+  if (best) *best = cand[pick];
+  if (ncand) {
+    const int n = std::min<int>(*ncand, (int)cand.size());
+    for (int k = 0; k < n; ++k) {
+      if (widths) widths[k] = cand[k];
+      if (us_per_probe_slice) us_per_probe_slice[k] = cost[k];
+    }
+    *ncand = (int)cand.size();
+  }
+  API_END(s)
+}
+
 msl_status msl_last_timings(const msl_sim* s, float* out4) {
   if (!s || !out4) return MSL_ERR_ARG;
   std::copy(s->timings, s->timings + 4, out4);
@@ -1199,8 +1253,12 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
   const int nslices = d->nslices > 0 ? d->nslices : (int)std::ceil((s->MaxZ - s->MinZ) / d->dz);
   const int nconf = std::max(1, d->nconfig);
   const bool displace = d->sigma > 0.f;
-  // probes per launch: all of them when they fit in ~16 GiB of waves
-  int B = d->batch > 0 ? d->batch : (int)std::min<long long>(d->nprobe, (16LL << 30) / (np * 8));
+  // probes per launch: the tuned width for this grid (msl_autotune_batch), else
+  // all of them when they fit in ~16 GiB of waves
+  const int auto_b = (s->tuned_batch > 0 && s->tuned_batch_n == nx)
+                         ? s->tuned_batch
+                         : (int)std::min<long long>(d->nprobe, (16LL << 30) / (np * 8));
+  int B = d->batch > 0 ? d->batch : auto_b;
   B = std::max(1, std::min(std::min(B, d->nprobe), 65535));  // grid.y limit of the column pass
   const int C = col_tile(ny), ntiles = nx / C;
   alloc_waves(s);

@@ -117,6 +117,8 @@ struct msl_sim {
   bool have_tds = false, have_image = false;
   float maxposIm = 0, maxposEW = 0, minposEW = 0, minposIm = 0, imagecontrast = 0;
   float timings[4] = {0, 0, 0, 0};
+  int tuned_batch = 0;  // probes per launch chosen by msl_autotune_batch (0: not tuned)
+  int tuned_batch_n = 0;  // grid size it was tuned for
   // per-kernel HIP-event timing of batched runs: [row ms, row n, col ms,
   // col n, detect ms, detect n, potential ms, potential n]
   bool profiling = false;

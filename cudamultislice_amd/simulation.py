@@ -247,6 +247,24 @@ class MultisliceSimulation:
             out["exit"] = ew
         return out
 
+    def autotune_batch(self, posx, posy, dz, nslices=8, adf=(60.0, 200.0)):
+        """Times one chunk of B of the given probes over nslices slices for
+        B = 64, 128, ... and keeps the fastest width (per probe-slice) for later
+        stem_batched(batch=0) runs on this grid; returns (best, {B: us per
+        probe-slice})."""
+        posx = np.ascontiguousarray(posx, np.int32)
+        posy = np.ascontiguousarray(posy, np.int32)
+        adf_out = np.zeros(posx.size, np.float32)
+        d = msl_stem_batch_desc(posx.size, _p(posx), _p(posy), 1, 0, 0.0, dz, nslices, adf[0], adf[1], 0, 0,
+                                _p(adf_out), None, None)
+        best, n = C.c_int(0), C.c_int(32)
+        widths = np.zeros(32, np.int32)
+        cost = np.zeros(32, np.float32)
+        self._c(lib().msl_autotune_batch(self._h, C.byref(d), int(nslices), C.byref(best), _p(widths), _p(cost),
+                                         C.byref(n)))
+        k = min(n.value, 32)
+        return best.value, {int(w): float(c) for w, c in zip(widths[:k], cost[:k])}
+
     def set_profiling(self, on: bool = True):
         self._c(lib().msl_set_profiling(self._h, 1 if on else 0))
 

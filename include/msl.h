@@ -192,6 +192,13 @@ typedef struct {
   float* exit_out;        /* optional [nconfig][nprobe][resY][resX][2] exit waves (small runs) */
 } msl_stem_batch_desc;
 msl_status msl_stem_batched(msl_sim* sim, const msl_stem_batch_desc* desc);
+/* batch width autotuner: runs one chunk of B of the descriptor's probes over
+ * nslices slices for B = 64, 128, ... up to nprobe (and ~32 GiB of waves),
+ * and keeps the fastest per probe-slice as the width of later batch = 0 runs
+ * on this grid size.  On input *ncand is the capacity of widths /
+ * us_per_probe_slice (may be 0); on output the number of candidates. */
+msl_status msl_autotune_batch(msl_sim* sim, const msl_stem_batch_desc* desc, int nslices, int* best, int* widths,
+                              float* us_per_probe_slice, int* ncand);
 
 /* time spent in the last msl_stem_batched, split by stage (milliseconds):
  * [0] binning, [1] potentials, [2] propagation, [3] total */

@@ -474,6 +474,24 @@ def test_every_pass_variant_matches_oracle_in_batched_stem(fp):
         L.msl_set_pass_variant(1, N, 0)
 
 
+def test_batch_width_autotuner_keeps_results(fp):
+    """msl_autotune_batch times the candidate widths and keeps the fastest;
+    a batch = 0 run then uses it and gives the bits of a single-launch run
+    (each probe's arithmetic does not depend on the chunk it is in)."""
+    Z, x, y, z = S.srtio3(3, 3, 4)
+    g, o = _both(Z, x, y, z, fp)
+    _setup(g, o, 300.0, 0.2, 64, 20.0, stem=True, obj=25.0)
+    rng = np.random.default_rng(5)
+    px = rng.integers(0, 64, 300).astype(np.int32)
+    py = rng.integers(0, 64, 300).astype(np.int32)
+    best, cost = g.autotune_batch(px, py, 2.0, nslices=4)
+    assert sorted(cost) == [64, 128, 256, 300] and best in cost
+    assert all(c > 0 for c in cost.values())
+    a = g.stem_batched(px, py, 2.0, nconfig=1, adf=(10.0, 200.0))["adf"]
+    b = g.stem_batched(px, py, 2.0, nconfig=1, adf=(10.0, 200.0), batch=300)["adf"]
+    assert np.array_equal(a, b)
+
+
 def test_batched_run_is_deterministic(fp):
     Z, x, y, z = S.srtio3(3, 3, 4)
     g, _ = _both(Z, x, y, z, fp)
