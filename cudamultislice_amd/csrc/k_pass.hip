@@ -760,34 +760,36 @@ constexpr int kGroupRowLds = cmax(cmax(group_row_lds<256>(), group_row_lds<512>(
 constexpr int kGroupColLds = cmax(cmax(GroupCol<256>::LDS, GroupCol<512>::LDS), cmax(GroupCol<1024>::LDS, GroupCol<2048>::LDS));
 constexpr int kGroupTw = 2048;
 
-// Segment of workgroup `it` and its twiddles staged in LDS.  The table lives
-// in device memory; its fields are read with a uniform index.  (A persistent
-// loop over the blocks of all segments was measured by the compiler at 256
-// VGPRs with the four bodies in one loop: one workgroup per block keeps the
-// register count of the largest body.)
-template <class A>
-__device__ __forceinline__ int group_seg(const PassGroup<A>* g, long long it, float2* stw, int nthreads) {
-  const int nseg = g->nseg;
-  int s = 0;
-  while (s + 1 < nseg && it >= g->blk0[s + 1]) ++s;
-  const float2* tw = reinterpret_cast<const typename GlobalView<A>::T*>(&g->a[s])->tw;
-  const int n = g->a[s].nx;
-  for (int i = threadIdx.x; i < n; i += nthreads) stw[i] = tw[i];
-  __syncthreads();
-  return s;
+// Segment boundaries and lengths as plain kernel arguments: the segment of a
+// workgroup and its length come from scalar compares and selects, and only
+// the segment's pass arguments are read from the table in device memory
+// (one level of dependent loads before the body, as for a per-size pass).
+// (A persistent loop over the blocks of all segments was measured by the
+// compiler at 256 VGPRs with the four bodies in one loop: one workgroup per
+// block keeps the register count of the largest body.)
+struct GroupHdr {
+  long long b1, b2, b3, total;  // first block of segments 1..3 (= total when absent)
+  int n0, n1, n2, n3;           // lengths
+  int x0, x1, x2, x3;           // columns: launched tiles per wave
+};
+template <class T> __device__ __forceinline__ T pick4(int s, T a, T b, T c, T d) {
+  return s == 0 ? a : s == 1 ? b : s == 2 ? c : d;
 }
 
 template <int MODE, int MUL, int FL>
-__global__ void __launch_bounds__(256) k_row_group(const PassGroup<RowArgs>* g) {
+__global__ void __launch_bounds__(256) k_row_group(const PassGroup<RowArgs>* g, GroupHdr h) {
   __shared__ float2 lds[kGroupRowLds];
   __shared__ float2 stw[kGroupTw];
   // XCD-aware, and at 2048 wave-interleaved inside the segment (as k_row)
   const long long it = xcd_remap(blockIdx.x, gridDim.x);
-  if (it >= g->blk0[g->nseg]) return;
-  const int s = group_seg(g, it, stw, 256);
+  if (it >= h.total) return;
+  const int s = (it >= h.b1) + (it >= h.b2) + (it >= h.b3);
+  const long long blk = it - pick4<long long>(s, 0, h.b1, h.b2, h.b3);
+  const int n = pick4(s, h.n0, h.n1, h.n2, h.n3);
   const RowArgsG& a = reinterpret_cast<const RowArgsG*>(g->a)[s];
-  const long long blk = it - g->blk0[s];
-  switch (a.nx) {
+  for (int i = threadIdx.x; i < n; i += 256) stw[i] = a.tw[i];
+  __syncthreads();
+  switch (n) {
     case 256: row_block<256, MODE, MUL, FL, GroupRow<256>::RV>(a, blk, lds, stw); break;
     case 512: row_block<512, MODE, MUL, FL, GroupRow<512>::RV>(a, blk, lds, stw); break;
     case 1024: row_block<1024, MODE, MUL, FL, GroupRow<1024>::RV>(a, blk, lds, stw); break;
@@ -800,23 +802,43 @@ __global__ void __launch_bounds__(256) k_row_group(const PassGroup<RowArgs>* g) 
 }
 
 template <int MODE, int MUL>
-__global__ void __launch_bounds__(512) k_col_group(const PassGroup<ColArgs>* g) {
+__global__ void __launch_bounds__(512) k_col_group(const PassGroup<ColArgs>* g, GroupHdr h) {
   __shared__ float2 lds[kGroupColLds];
   __shared__ float2 stw[kGroupTw];
   // XCD-aware: each XCD takes a contiguous range of blocks, so neighbouring
   // tiles share an L2
   const long long it = xcd_remap(blockIdx.x, gridDim.x);
-  if (it >= g->blk0[g->nseg]) return;
-  const int s = group_seg(g, it, stw, 512);
+  if (it >= h.total) return;
+  const int s = (it >= h.b1) + (it >= h.b2) + (it >= h.b3);
+  const long long blk = it - pick4<long long>(s, 0, h.b1, h.b2, h.b3);
+  const int n = pick4(s, h.n0, h.n1, h.n2, h.n3), nbx = pick4(s, h.x0, h.x1, h.x2, h.x3);
+  const int b = (int)(blk / nbx), bx = (int)(blk - (long long)b * nbx);
   const ColArgsG& a = reinterpret_cast<const ColArgsG*>(g->a)[s];
-  const long long blk = it - g->blk0[s];
-  const int nbx = g->nbx[s], b = (int)(blk / nbx), bx = (int)(blk - (long long)b * nbx);
-  switch (a.nx) {
+  for (int i = threadIdx.x; i < n; i += 512) stw[i] = a.tw[i];
+  __syncthreads();
+  switch (n) {
     case 256: col_block<256, MODE, MUL, GroupCol<256>>(a, bx, b, nbx, lds, stw); break;
     case 512: col_block<512, MODE, MUL, GroupCol<512>>(a, bx, b, nbx, lds, stw); break;
     case 1024: col_block<1024, MODE, MUL, GroupCol<1024>>(a, bx, b, nbx, lds, stw); break;
     default: col_block<2048, MODE, MUL, GroupCol<2048>>(a, bx, b, nbx, lds, stw); break;
   }
+}
+
+template <class A> static GroupHdr group_hdr(const PassGroup<A>& h) {
+  GroupHdr r{};
+  const long long total = h.blk0[h.nseg];
+  r.total = total;
+  r.b1 = h.nseg > 1 ? h.blk0[1] : total;
+  r.b2 = h.nseg > 2 ? h.blk0[2] : total;
+  r.b3 = h.nseg > 3 ? h.blk0[3] : total;
+  int n[4] = {0, 0, 0, 0}, x[4] = {1, 1, 1, 1};
+  for (int k = 0; k < h.nseg; ++k) {
+    n[k] = h.a[k].nx;
+    x[k] = h.nbx[k] > 0 ? h.nbx[k] : 1;
+  }
+  r.n0 = n[0]; r.n1 = n[1]; r.n2 = n[2]; r.n3 = n[3];
+  r.x0 = x[0]; r.x1 = x[1]; r.x2 = x[2]; r.x3 = x[3];
+  return r;
 }
 
 static bool group_len(int n) { return n == 256 || n == 512 || n == 1024 || n == 2048; }
@@ -885,7 +907,8 @@ hipError_t launch_row_group(int mode, int mul, const PassGroup<RowArgs>& h, cons
   if (!d) return hipErrorInvalidValue;
   const int fl = (h.a[0].prune_in ? PRUNE_IN : 0) | (h.a[0].prune_out ? PRUNE_OUT : 0);
   if (total > 0x7fffffffLL) return hipErrorInvalidValue;
-#define MSL_RG(M, U, F) hipLaunchKernelGGL((k_row_group<M, U, F>), dim3((unsigned)total), dim3(256), 0, s, d)
+  const GroupHdr hd = group_hdr(h);
+#define MSL_RG(M, U, F) hipLaunchKernelGGL((k_row_group<M, U, F>), dim3((unsigned)total), dim3(256), 0, s, d, hd)
   if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_RG(ROW_INV_MUL_FWD, RM_T, PRUNE_OUT);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT))
     MSL_RG(ROW_INV_MUL_FWD, RM_T, PRUNE_IN | PRUNE_OUT);
@@ -900,7 +923,8 @@ hipError_t launch_col_group(int mode, int mul, const PassGroup<ColArgs>& h, cons
   if (h.nseg == 0 || total <= 0) return hipSuccess;
   if (!d) return hipErrorInvalidValue;
   if (total > 0x7fffffffLL) return hipErrorInvalidValue;
-#define MSL_CG(M, U) hipLaunchKernelGGL((k_col_group<M, U>), dim3((unsigned)total), dim3(512), 0, s, d)
+  const GroupHdr hd = group_hdr(h);
+#define MSL_CG(M, U) hipLaunchKernelGGL((k_col_group<M, U>), dim3((unsigned)total), dim3(512), 0, s, d, hd)
   if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_CG(COL_FWD_MUL_INV, CM_PROP);
   else return hipErrorInvalidValue;
 #undef MSL_CG
