@@ -299,25 +299,32 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
       const long long o = (long long)b * a.dst_bstride + (long long)y * N;
       const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* vr = a.t + slot * a.t_bstride + (long long)y * N;
-      float2 pm[E], p0[E];
-      float vv[E];
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        pm[c] = a.fd_pm[o + t + c * T];
-        p0[c] = a.fd_p0[o + t + c * T];
-        vv[c] = vr[t + c * T].x;
-      }
       const float dz = a.fd_dz, wavel = a.fd_wavel;
       const float al = 2.f * 3.14159f * dz / wavel;
       const float den = 1.f + al * al;
       const float2 rcp = make_float2(1.f / den, -al / den);                    // 1 / (1 + i a)
       const float2 ratio = make_float2((1.f - al * al) / den, -2.f * al / den);  // (1 - i a) / (1 + i a)
+      // in groups of Q elements: the loads of one group are issued together,
+      // and fewer of them are live at once
+      constexpr int Q = E > 8 ? 8 : E;
 #pragma unroll
-      for (int c = 0; c < E; ++c) {
-        const float2 p = p0[c], g = v[c];
-        const float w = dz * dz * 4.f * 3.14159f * a.sigma * vv[c] / wavel;
-        const float2 A = make_float2(2.f * p.x - dz * dz * g.x - w * p.x, 2.f * p.y - dz * dz * g.y - w * p.y);
-        v[c] = csub(cmul(rcp, A), cmul(ratio, pm[c]));
+      for (int c0 = 0; c0 < E; c0 += Q) {
+        float2 pm[Q], p0[Q];
+        float vv[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          pm[q] = a.fd_pm[o + t + (c0 + q) * T];
+          p0[q] = a.fd_p0[o + t + (c0 + q) * T];
+          vv[q] = vr[t + (c0 + q) * T].x;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float2 p = p0[q], g = v[c0 + q];
+          const float w = dz * dz * 4.f * 3.14159f * a.sigma * vv[q] / wavel;
+          const float2 A = make_float2(2.f * p.x - dz * dz * g.x - w * p.x, 2.f * p.y - dz * dz * g.y - w * p.y);
+          v[c0 + q] = csub(cmul(rcp, A), cmul(ratio, pm[q]));
+        }
+        asm volatile("" ::: "memory");
       }
     }
   } else if constexpr (MUL == RM_T) {
