@@ -428,9 +428,16 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         v, t = cpu_baseline(atoms, px, py, args.cpu_probes, args.cpu_slices, threads)
+        model = None
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+        except OSError:
+            pass
         cpu = {"value": v, "unit": "probe*slice/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_probes} probes x {args.cpu_slices} slices x 1 configuration of the same "
-                         f"workload (potentials for those slices included), float restatement, {t:.1f} s"}
+                         f"workload (potentials for those slices included), float restatement, {t:.1f} s",
+               "host_cpus": os.cpu_count(), "cpu_model": model}
         log(f"[bench] cpu baseline {v:.4g} probe*slice/s on {threads} threads ({t:.1f} s)")
 
     out = {
