@@ -705,6 +705,41 @@ def test_diff_checkpoint_interval_does_not_change_gradient():
     assert rel_l2(grads[1], grads[0]) < 1e-6 and rel_l2(grads[2], grads[0]) < 1e-6
 
 
+def test_invalid_calls_return_status_codes(fp):
+    """Error behaviour of the C-ABI (the reference has none: void methods,
+    ignored CUDA statuses, exit(1)): every misuse below returns a status and a
+    message, leaves the handle usable, and launches nothing."""
+    from cudamultislice_amd import MultisliceSimulation
+    from cudamultislice_amd._lib import MslError
+    from cudamultislice_amd.serve import Job, serve
+    ARG, STATE, UNSUPPORTED = 1, 2, 5
+
+    def code(f, *a, **k):
+        with pytest.raises(MslError) as e:
+            f(*a, **k)
+        return e.value.code
+
+    Z, x, y, z = S.srtio3(2, 2, 2)
+    with MultisliceSimulation() as g:
+        assert code(g.set_atoms, Z[:0], x[:0], y[:0], z[:0]) == ARG          # empty structure
+        assert code(g.MultisliceStepConv, 1) == STATE                          # step before set-up
+        g.set_atoms(Z, x, y, z)
+        g.GetParameterisation()
+        g.UploadBinnedAtoms()
+        g.ApplyMicroscopeParameters(300.0)
+        assert code(g.SetCalculationVariables, 0.2, 0.0, 1000, 1000, 1, 1, 8.0) == UNSUPPORTED  # not 2^k
+        assert code(g.SetCalculationVariables, 0.2, 0.0, 4096, 4096, 1, 1, 8.0) == UNSUPPORTED  # > 2048
+        g.SetCalculationVariables(0.2, 0.0, 64, 64, 1, 1, 8.0)
+        g.PreCalculateFrequencies()
+        assert code(g.MultisliceStepConv, 1) == STATE                          # before InitialiseWavefunctions
+        g.InitialiseWavefunctions(2.0)
+        g.MultisliceStepConv(1)                                                # the handle still works
+        assert code(g.stem_batched, np.zeros(0, np.int32), np.zeros(0, np.int32), 2.0) == ARG
+    bad = Job(64, Z, x, y, z, nslices=2, posx=np.array([64], np.int32), posy=np.array([0], np.int32))
+    assert code(serve, [bad]) == ARG                                            # probe outside the grid
+    assert code(serve, [Job(100, Z, x, y, z, nslices=2)]) == UNSUPPORTED        # grid not 2^k
+
+
 def test_handles_release_their_device_memory(fp):
     """Every handle frees what it allocated (the reference leaks its plans
     and frequency arrays on each PreCalculateFrequencies): device memory in
