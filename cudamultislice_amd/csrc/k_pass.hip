@@ -180,7 +180,8 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
     }
   }
   const LL L{lds + ln * LS, 1};
-  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S>(v, t, stw, L);
+  constexpr bool ZIN = (FL & PRUNE_IN) != 0 && E % 4 == 0 && P::R1 % 4 == 0;  // first transform's input
+  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S, LL, ZIN>(v, t, stw, L);
   if constexpr ((FL & STORE_AUX) != 0) {
     // read back only much later (by the adjoint pass): streamed past the caches
     if (ok) {
@@ -218,7 +219,8 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
       for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
     }
   }
-  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
+  if constexpr (MODE == ROW_FWD) fft_line<P, N, -1, S, LL, ZIN>(v, t, stw, L);
+  if constexpr (MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
   if (ok) {
     float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
@@ -269,26 +271,39 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
   using LL = LdsLine<Cfg::SH>;
   constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
-  const long long line = blk * LINES + ln;
-  const bool ok = line < (long long)a.nbatch * a.ny;
-  const int b = ok ? (int)(line / a.ny) : 0;
-  const int y = ok ? (int)(line - (long long)b * a.ny) : 0;
+  // The block's first line splits into wave b0 and row on the scalar unit
+  // (the block index is uniform; the grid holds no block past the last
+  // line, so b0 < nbatch); a thread's own line is db waves further (db = 0
+  // unless LINES straddles a wave), with no per-thread 64-bit division.
+  const long long first = blk * LINES;
+  const long long b0 = first / a.ny;
+  int y = (int)(first - b0 * a.ny) + ln;
+  int db = 0;
+  while (y >= a.ny) {  // at most LINES / ny + 1 steps; none when ny % LINES == 0
+    y -= a.ny;
+    ++db;
+  }
+  const bool ok = b0 + db < a.nbatch;
+  const int b = ok ? (int)b0 + db : 0;
+  // element offset of this thread's row in an array of waves of stride bs
+  auto roff = [&](long long bs) { return b0 * bs + ((long long)db * bs + (long long)y * N); };
   float2 v[E];
-  {
-    const float2* row = a.src + (long long)b * a.src_bstride + (long long)y * N;
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const bool load = ok && (!(FL & PRUNE_IN) || BandBlocks<E>::active(c));
-      v[c] = load ? row[t + c * T] : make_float2(0.f, 0.f);
-    }
+  for (int c = 0; c < E; ++c) v[c] = make_float2(0.f, 0.f);
+  if (ok) {
+    const float2* row = a.src + roff(a.src_bstride);
+#pragma unroll
+    for (int c = 0; c < E; ++c)
+      if (!(FL & PRUNE_IN) || BandBlocks<E>::active(c)) v[c] = row[t + c * T];
   }
   const LL L{lds + ln * LS, 1};
+  constexpr bool ZIN = (FL & PRUNE_IN) != 0 && E % 4 == 0 && P::R1 % 4 == 0;  // first transform's input
   if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD || MODE == ROW_INV_FD_FWD)
-    fft_line<P, N, +1, S>(v, t, stw, L);
+    fft_line<P, N, +1, S, LL, ZIN>(v, t, stw, L);
   if constexpr ((FL & STORE_AUX) != 0) {
     // read back only much later (by the adjoint pass): streamed past the caches
     if (ok) {
-      float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
+      float2* row = a.aux + roff(a.dst_bstride);
 #pragma unroll
       for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
     }
@@ -296,7 +311,7 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
   if constexpr (MODE == ROW_INV_FD_FWD) {
     // finiteDifferenceKernel, U.cu:995-1023: v holds the Laplacian G of psi
     if (ok) {
-      const long long o = (long long)b * a.dst_bstride + (long long)y * N;
+      const long long o = roff(a.dst_bstride);
       const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* vr = a.t + slot * a.t_bstride + (long long)y * N;
       const float dz = a.fd_dz, wavel = a.fd_wavel;
@@ -329,7 +344,9 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
     }
   } else if constexpr (MUL == RM_T) {
     if (ok) {
-      const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
+      // slot of wave b0 on the scalar unit; per thread only past a wave edge
+      long long slot = a.t_index ? (long long)a.t_index[b0] : b0 / a.t_group;
+      if (db != 0) slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
 #pragma unroll
       for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
@@ -356,10 +373,10 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
       for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
     }
   }
-  if constexpr (MODE == ROW_FWD || MODE == ROW_INV_MUL_FWD || MODE == ROW_INV_FD_FWD)
-    fft_line<P, N, -1, S>(v, t, stw, L);
+  if constexpr (MODE == ROW_FWD) fft_line<P, N, -1, S, LL, ZIN>(v, t, stw, L);
+  if constexpr (MODE == ROW_INV_MUL_FWD || MODE == ROW_INV_FD_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
   if (ok) {
-    float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
+    float2* row = a.dst + roff(a.dst_bstride);
 #pragma unroll
     for (int c = 0; c < E; ++c)
       if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];

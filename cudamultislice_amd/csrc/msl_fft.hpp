@@ -47,7 +47,19 @@ template <int DIR> __host__ __device__ __forceinline__ void dft2(float2& a0, flo
   a1 = csub(t, a1);
 }
 
-template <int DIR> __host__ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+// Z: a1 = a2 = 0 on entry (the middle of a band-limited line).  Adding an
+// exact zero changes nothing but the sign of a zero result, so the shortened
+// butterfly gives the values of the full one.
+template <int DIR, bool Z = false>
+__host__ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  if constexpr (Z) {
+    const float2 x = a0, r = rot4<DIR>(a3);
+    a0 = cadd(x, a3);
+    a2 = csub(x, a3);
+    a1 = csub(x, r);
+    a3 = cadd(x, r);
+    return;
+  }
   float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = rot4<DIR>(csub(a1, a3));
   a0 = cadd(t0, t2);
   a2 = csub(t0, t2);
@@ -55,12 +67,13 @@ template <int DIR> __host__ __device__ __forceinline__ void dft4(float2& a0, flo
   a3 = csub(t1, t3);
 }
 
-template <int DIR> __host__ __device__ __forceinline__ void dft8(float2 (&a)[8]) {
+// Z: a[2..5] = 0 on entry
+template <int DIR, bool Z = false> __host__ __device__ __forceinline__ void dft8(float2 (&a)[8]) {
   constexpr float h = 0.70710678118654752f;
   float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
   float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
-  dft4<DIR>(e0, e1, e2, e3);
-  dft4<DIR>(o0, o1, o2, o3);
+  dft4<DIR, Z>(e0, e1, e2, e3);
+  dft4<DIR, Z>(o0, o1, o2, o3);
   // o1 *= W8^1, o2 *= W8^2, o3 *= W8^3 with W8 = exp(DIR 2 pi i / 8)
   if (DIR < 0) {
     o1 = make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
@@ -76,11 +89,12 @@ template <int DIR> __host__ __device__ __forceinline__ void dft8(float2 (&a)[8])
   a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
 }
 
-template <int DIR> __host__ __device__ __forceinline__ void dft16(float2 (&a)[16]) {
+// Z: a[4..11] = 0 on entry
+template <int DIR, bool Z = false> __host__ __device__ __forceinline__ void dft16(float2 (&a)[16]) {
   // 16 = 4 x 4: DFT4 over n1 (stride 4), twiddle W16^(n2 k1), DFT4 over n2
   constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
 #pragma unroll
-  for (int n2 = 0; n2 < 4; ++n2) dft4<DIR>(a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12]);
+  for (int n2 = 0; n2 < 4; ++n2) dft4<DIR, Z>(a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12]);
   // now a[n2 + 4 k1] = B[n2][k1]; apply W16^(n2*k1), W16 = exp(DIR 2 pi i/16)
   const float sg = DIR < 0 ? -1.f : 1.f;
   const float2 w1 = make_float2(c1, sg * s1), w2 = make_float2(h, sg * h), w3 = make_float2(s1, sg * c1);
@@ -106,11 +120,13 @@ template <int DIR> __host__ __device__ __forceinline__ void dft16(float2 (&a)[16
   for (int i = 0; i < 16; ++i) a[i] = o[i];
 }
 
-template <int R, int DIR> __host__ __device__ __forceinline__ void dftR(float2 (&a)[R]) {
+// Z: a[R/4 .. 3R/4 - 1] = 0 on entry (R >= 4)
+template <int R, int DIR, bool Z = false> __host__ __device__ __forceinline__ void dftR(float2 (&a)[R]) {
+  static_assert(!Z || R >= 4, "zero-middle butterflies need R >= 4");
   if constexpr (R == 2) dft2<DIR>(a[0], a[1]);
-  else if constexpr (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
-  else if constexpr (R == 8) dft8<DIR>(a);
-  else if constexpr (R == 16) dft16<DIR>(a);
+  else if constexpr (R == 4) dft4<DIR, Z>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<DIR, Z>(a);
+  else if constexpr (R == 16) dft16<DIR, Z>(a);
 }
 
 // Per-length plan: T threads per line, radices of up to three stages.
@@ -181,11 +197,14 @@ template <class LL> struct DualLine {
 // x[j + m N/R] = v[b + m NB]; outputs go to y[(j/L) L R + (j%L) + m L].
 // If LAST, outputs stay in registers in canonical order; otherwise they are
 // written to LDS, and read back in canonical order after a barrier.
-template <int N, int T, int R, int L, int DIR, bool LAST, class S, int BUF, class LL>
+// ZIN (first stage only): v[c] = 0 for E/4 <= c < 3E/4, so every butterfly
+// gets zeros at m = R/4 .. 3R/4 - 1 (c = b + m NB with b < NB).
+template <int N, int T, int R, int L, int DIR, bool LAST, class S, int BUF, class LL, bool ZIN = false>
 __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const float2* __restrict__ tw,
                                                const LL& lds) {
   constexpr int E = N / T, NB = E / R;
   static_assert(E % R == 0, "radix must divide the per-thread count");
+  static_assert(!ZIN || (L == 1 && R % 4 == 0), "zero-middle input only in a first stage of radix 4k");
   float2 out[E];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -203,7 +222,7 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
         a[m] = cmul(a[m], w);
       }
     }
-    dftR<R, DIR>(a);
+    dftR<R, DIR, ZIN>(a);
     if constexpr (LAST) {
 #pragma unroll
       for (int m = 0; m < R; ++m) out[b + m * NB] = a[m];
@@ -227,15 +246,16 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
 // Full transform of one line with plan P (T threads, radices R1 R2 R3).
 // tw points at the N-entry table exp(-2 pi i k / N).  Every thread that
 // shares the synchronisation scope must call this together.
-template <class P, int N, int DIR, class S = BlockSync, class LL = LdsLine<0>>
+// ZIN: the line is band-limited, v[c] = 0 for E/4 <= c < 3E/4 (E = N / T).
+template <class P, int N, int DIR, class S = BlockSync, class LL = LdsLine<0>, bool ZIN = false>
 __device__ __forceinline__ void fft_line(float2 (&v)[N / P::T], int t, const float2* __restrict__ tw, const LL& lds) {
   constexpr int T = P::T;
   if constexpr (P::R3 > 1) {  // two exchanges: copies 0, 1 in every transform
-    stockham_stage<N, T, P::R1, 1, DIR, false, S, 0>(v, t, tw, lds);
+    stockham_stage<N, T, P::R1, 1, DIR, false, S, 0, LL, ZIN>(v, t, tw, lds);
     stockham_stage<N, T, P::R2, P::R1, DIR, false, S, 1>(v, t, tw, lds);
     stockham_stage<N, T, P::R3, P::R1 * P::R2, DIR, true, S, 0>(v, t, tw, lds);
   } else {  // one exchange: the copy follows the direction
-    stockham_stage<N, T, P::R1, 1, DIR, false, S, (DIR < 0 ? 1 : 0)>(v, t, tw, lds);
+    stockham_stage<N, T, P::R1, 1, DIR, false, S, (DIR < 0 ? 1 : 0), LL, ZIN>(v, t, tw, lds);
     stockham_stage<N, T, P::R2, P::R1, DIR, true, S, 0>(v, t, tw, lds);
   }
 }
