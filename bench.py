@@ -80,7 +80,10 @@ def diff_ratio(reps=3):
     projected potential of amorphous carbon (1.2 g/cm^3, 102.4x102.4x200 A,
     min distance 1.4 A), 64 probes (8x8 raster, 25 mrad, 300 kV), loss
     sum (|psi|^2 - I_target)^2 with I_target from a seeded +-2 % perturbed
-    potential, checkpoint every 10 slices."""
+    potential.  Timed twice on the same inputs: with every slice's wave kept
+    in HBM (checkpoint_every = 0: 51.5 GiB of workspace, no recomputation;
+    the headline ratio) and with checkpoints every 10 slices (11 GiB, each
+    segment recomputed once in the backward)."""
     import numpy as np
     import torch
     from cudamultislice_amd import MultisliceSimulation, POT_PROJECTED
@@ -113,13 +116,32 @@ def diff_ratio(reps=3):
     Vt = torch.from_numpy(V).cuda()
     Vp = torch.from_numpy((V * (1 + 0.02 * rng.uniform(-1, 1, V.shape))).astype(np.float32)).cuda()
     p0 = torch.from_numpy(psi0).cuda()
-    eng = DiffEngine(N, N, ps, 300.0, dz, ns, B, checkpoint_every=10)
     with torch.no_grad():
+        eng = DiffEngine(N, N, ps, 300.0, dz, ns, B, checkpoint_every=10)
         target = eng.forward(Vp, p0).clone()
+        eng.close()
+    res = {}
+    for K in (0, 10):
+        eng = DiffEngine(N, N, ps, 300.0, dz, ns, B, checkpoint_every=K)
+        res[K] = _diff_time(eng, Vt, p0, target, reps)
+        eng.close()
+        log(f"[diff] checkpoint_every={K}: fwd {res[K][0]:.1f} ms, fwd+bwd {res[K][1]:.1f} ms")
+    tf, tfb, ws = res[0]
+    tf10, tfb10, ws10 = res[10]
+    return {"fwd_ms": tf, "fwd_bwd_ms": tfb, "ratio": tfb / tf, "grid": N, "slices": ns, "probes": B,
+            "checkpoint_every": 0, "workspace_GiB": ws / 2**30,
+            "checkpoint_every_10": {"fwd_ms": tf10, "fwd_bwd_ms": tfb10, "ratio": tfb10 / tf10,
+                                    "workspace_GiB": ws10 / 2**30}}
 
-    def fwd():
+
+def _diff_time(eng, Vt, p0, target, reps):
+    """(forward ms, forward+backward ms, workspace bytes) of one engine"""
+    import torch
+    from cudamultislice_amd.diff import MultisliceIntensity
+
+    def fwd():  # inference forward: nothing kept for a backward
         with torch.no_grad():
-            I = eng.forward(Vt, p0)
+            I = eng.forward(Vt, p0, keep=False)
             return ((I - target) ** 2).sum()
 
     def fwd_bwd():
@@ -142,10 +164,7 @@ def diff_ratio(reps=3):
     torch.cuda.synchronize()
     tfb = (time.perf_counter() - t0) / reps
     assert torch.isfinite(g).all()
-    ws = eng.info()["workspace_bytes"]
-    eng.close()
-    return {"fwd_ms": tf * 1e3, "fwd_bwd_ms": tfb * 1e3, "ratio": tfb / tf, "grid": N, "slices": ns, "probes": B,
-            "checkpoint_every": 10, "workspace_GiB": ws / 2**30}
+    return tf * 1e3, tfb * 1e3, eng.info()["workspace_bytes"]
 
 
 def tem_cases(cpu_threads):

@@ -84,6 +84,7 @@ _SIGS = {
     "msl_diff_create": ([C.POINTER(_vp), _i, _i, _i, _f, _f, _f, _i, _i, _i], _i),
     "msl_diff_destroy": ([_vp], None), "msl_diff_last_error": ([_vp], _cp),
     "msl_diff_forward": ([_vp, _vp, _vp, _vp, _vp], _i),
+    "msl_diff_forward_only": ([_vp, _vp, _vp, _vp, _vp], _i),
     "msl_diff_backward": ([_vp, _vp, _vp, _vp], _i),
     "msl_diff_info": ([_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(C.c_longlong)], _i),
     "msl_fft2_device": ([_i, _i, _i, _vp, _i, _vp], _i),

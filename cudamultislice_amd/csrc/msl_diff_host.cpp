@@ -201,7 +201,22 @@ msl_status msl_diff_info(const msl_diff* d, float* wl, float* sg, float* km, lon
   return MSL_OK;
 }
 
+namespace {
+// keep = false: intensities only; no checkpoints, no stored real-space waves,
+// and the backward refuses until a forward with keep = true
+msl_status diff_forward(msl_diff* d, const float* V, const void* psi0, float* inten, void* stream, bool keep);
+}  // namespace
+
 msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float* inten, void* stream) {
+  return diff_forward(d, V, psi0, inten, stream, true);
+}
+
+msl_status msl_diff_forward_only(msl_diff* d, const float* V, const void* psi0, float* inten, void* stream) {
+  return diff_forward(d, V, psi0, inten, stream, false);
+}
+
+namespace {
+msl_status diff_forward(msl_diff* d, const float* V, const void* psi0, float* inten, void* stream, bool keep) {
   if (!d || !V || !psi0 || !inten) return MSL_ERR_ARG;
   try {
     HC(hipSetDevice(d->device));
@@ -214,14 +229,15 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
     // waves w_i stay in seg for the backward pass
     const int last_base = (d->S - 1) * d->K;
     const float2* zprev = d->psi0;
+    d->have_forward = false;
     for (int i = 1; i <= d->ns; ++i) {
-      float2* dst = (i % d->K == 0 && i < d->ns) ? ckpt_slot(d, i) : d->W.p;
-      fwd_slice(d, i, zprev, dst, i > last_base ? wslot(d, i, last_base) : nullptr, s);
+      float2* dst = (keep && i % d->K == 0 && i < d->ns) ? ckpt_slot(d, i) : d->W.p;
+      fwd_slice(d, i, zprev, dst, keep && i > last_base ? wslot(d, i, last_base) : nullptr, s);
       zprev = dst;
     }
     HC(launch_row(ROW_INV, RM_NONE, rargs(d, zprev, d->exitw.p, true, false), s));
     HC(launch_intensity(d->exitw.p, inten, (long long)d->B * d->np(), 1.0f, s));
-    d->have_forward = true;
+    d->have_forward = keep;
   } catch (const DErr& e) {
     d->err = e.what();
     return e.code;
@@ -229,6 +245,7 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
   d->err.clear();
   return MSL_OK;
 }
+}  // namespace
 
 msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream) {
   if (!d || !dLdI || !gradV) return MSL_ERR_ARG;

@@ -54,12 +54,15 @@ class DiffEngine:
         if V.device.type != "cuda" or psi0.device.type != "cuda":
             raise ValueError("V and psi0 must be on the GPU")
 
-    def forward(self, V, psi0):
+    def forward(self, V, psi0, keep=True):
+        """intensities |psi_n|^2; keep=False stores nothing for a backward
+        (inference), after which backward raises until the next forward"""
         self._validate(V, psi0)
         out = torch.empty(self.shape, dtype=torch.float32, device=V.device)
         s = torch.cuda.current_stream(V.device).cuda_stream
-        self._check(lib().msl_diff_forward(self._h, C.c_void_p(V.data_ptr()), C.c_void_p(psi0.data_ptr()),
-                                           C.c_void_p(out.data_ptr()), C.c_void_p(s)))
+        f = lib().msl_diff_forward if keep else lib().msl_diff_forward_only
+        self._check(f(self._h, C.c_void_p(V.data_ptr()), C.c_void_p(psi0.data_ptr()), C.c_void_p(out.data_ptr()),
+                      C.c_void_p(s)))
         self._gen += 1
         return out
 

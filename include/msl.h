@@ -222,6 +222,11 @@ const char* msl_diff_last_error(const msl_diff* d);
  * null stream, ordered with all other default-stream work such as torch's
  * default stream) and not waited for */
 msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float* intensity, void* stream);
+/* the same intensities without keeping anything for a backward (no
+ * checkpoints, no stored real-space waves): inference, and the reference
+ * point of the backward's cost; msl_diff_backward then returns MSL_ERR_STATE
+ * until the next msl_diff_forward */
+msl_status msl_diff_forward_only(msl_diff* d, const float* V, const void* psi0, float* intensity, void* stream);
 msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream);
 msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, float* kmax, long long* workspace_bytes);
 

@@ -705,6 +705,33 @@ def test_diff_checkpoint_interval_does_not_change_gradient():
     assert rel_l2(grads[1], grads[0]) < 1e-6 and rel_l2(grads[2], grads[0]) < 1e-6
 
 
+def test_diff_forward_only_matches_forward_and_blocks_backward():
+    """The inference forward runs the same passes minus the stores kept for a
+    backward: the same intensities; a backward after it is refused, and the
+    next keeping forward restores the same gradient."""
+    import torch
+    from cudamultislice_amd._lib import MslError
+    from cudamultislice_amd.diff import DiffEngine
+    N, ns, B = 64, 7, 2
+    rng = np.random.default_rng(13)
+    V = torch.from_numpy((rng.random((ns, N, N)) * 20).astype(np.float32)).cuda()
+    p = torch.from_numpy((rng.standard_normal((B, N, N)) + 0j).astype(np.complex64)).cuda()
+    gI = torch.from_numpy(rng.standard_normal((B, N, N)).astype(np.float32)).cuda()
+    for K in (0, 3):
+        e = DiffEngine(N, N, 0.2, 200.0, 1.5, ns, B, checkpoint_every=K)
+        I_keep = e.forward(V, p).cpu().numpy()
+        g = e.backward(gI).cpu().numpy()
+        I_only = e.forward(V, p, keep=False).cpu().numpy()
+        # the keeping passes are separate kernels (they also store w_i): rounding only
+        assert rel_l2(I_only, I_keep) < 1e-6
+        with pytest.raises(MslError) as err:
+            e.backward(gI)
+        assert err.value.code == 2
+        e.forward(V, p)
+        assert np.array_equal(e.backward(gI).cpu().numpy(), g)
+        e.close()
+
+
 def test_invalid_calls_return_status_codes(fp):
     """Error behaviour of the C-ABI (the reference has none: void methods,
     ignored CUDA statuses, exit(1)): every misuse below returns a status and a
