@@ -114,7 +114,25 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     d->device = device;
     HC(hipSetDevice(device));
     d->nx = nx; d->ny = ny; d->ps = ps; d->kV = kV; d->dz = dz; d->ns = nslices; d->B = nprobe;
-    d->K = std::max(1, std::min(K <= 0 ? nslices : K, nslices));
+    if (K < 0) {
+      // the longest segment whose waves fit in half the free device memory:
+      // longer segments recompute fewer slices in the backward (none when
+      // K = nslices); the wave count 3 + (S - 1) + K is smallest near sqrt(ns)
+      size_t fr = 0, tot = 0;
+      HC(hipMemGetInfo(&fr, &tot));
+      const double wave = (double)nprobe * nx * ny * 8.0;
+      auto waves = [&](int k) { return 3.0 + (nslices + k - 1) / k - 1 + k; };
+      int best = 1;
+      for (int k = 1; k <= nslices; ++k)
+        if (waves(k) < waves(best)) best = k;
+      for (int k = nslices; k >= 1; --k)
+        if (waves(k) * wave <= 0.5 * (double)fr) {
+          best = std::max(best, k);
+          break;
+        }
+      K = best;
+    }
+    d->K = std::max(1, std::min(K == 0 ? nslices : K, nslices));
     d->S = (nslices + d->K - 1) / d->K;
     // FP32 physics of U.cu:1791-1799
     const float Pi = 3.1415926f, V = kV, e = 1.6e-019f;

@@ -3,8 +3,8 @@
 `MultisliceIntensity.apply(V, psi0, engine)` returns the exit-wave
 intensities |psi_n|^2 of a batch of probes propagated through per-slice
 potentials V (float32, [nslices, ny, nx], V*A); its backward returns dL/dV
-through the hand-written adjoint kernels of libmsl (checkpointed every
-`checkpoint_every` slices).  torch supplies device memory and the stream;
+through the hand-written adjoint kernels of libmsl (waves checkpointed every
+`checkpoint_every` slices, or all kept when they fit).  torch supplies device memory and the stream;
 all arithmetic runs in libmsl.
 """
 from __future__ import annotations
@@ -17,7 +17,10 @@ from ._lib import MslError, lib
 
 
 class DiffEngine:
-    def __init__(self, nx, ny, ps, kV, dz, nslices, nprobe, checkpoint_every=10, device=0):
+    def __init__(self, nx, ny, ps, kV, dz, nslices, nprobe, checkpoint_every=-1, device=0):
+        """checkpoint_every: K > 0 keeps every K-th wave and recomputes the rest
+        in the backward; 0 keeps every slice's wave; -1 (default) the longest
+        segment whose waves fit in half the free device memory"""
         self.shape = (nprobe, ny, nx)
         self.nslices = nslices
         self.device = device

@@ -213,7 +213,11 @@ msl_status msl_kernel_stats(const msl_sim* sim, double* out8);
 /* ---- differentiable multislice (no counterpart in the reference) ---- */
 typedef struct msl_diff msl_diff;
 /* psi_i = IFFT(P . FFT(exp(i sigma V_i) psi_{i-1})) / (nx ny), P the Fresnel
- * propagator of U.cu:399-428 for slice thickness dz; V given per slice. */
+ * propagator of U.cu:399-428 for slice thickness dz; V given per slice.
+ * checkpoint_every K: the forward keeps every K-th wave and the backward
+ * recomputes the others segment by segment; 0 keeps every slice's wave (no
+ * recomputation; (3 + nslices) x nprobe x nx x ny x 8 bytes); < 0 picks the
+ * longest segment whose waves fit in half the device's free memory. */
 msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps, float kV, float dz, int nslices,
                            int nprobe, int checkpoint_every);
 void msl_diff_destroy(msl_diff* d);

@@ -703,6 +703,12 @@ def test_diff_checkpoint_interval_does_not_change_gradient():
             assert np.array_equal(e.backward(gI).cpu().numpy(), grads[-1])
     # other intervals move the real-space waves through other passes (rounding only)
     assert rel_l2(grads[1], grads[0]) < 1e-6 and rel_l2(grads[2], grads[0]) < 1e-6
+    # the automatic interval keeps every wave of a problem this small: the
+    # workspace and the bits of checkpoint_every = nslices
+    e = DiffEngine(N, N, 0.2, 200.0, 1.5, ns, B, checkpoint_every=-1)
+    assert e.info()["workspace_bytes"] == DiffEngine(N, N, 0.2, 200.0, 1.5, ns, B, 0).info()["workspace_bytes"]
+    e.forward(V, p)
+    assert np.array_equal(e.backward(gI).cpu().numpy(), grads[2])
 
 
 def test_diff_forward_only_matches_forward_and_blocks_backward():
