@@ -177,6 +177,11 @@ template <int SH> struct LdsLine {
   static __host__ __device__ constexpr int pad(int p) { return SH > 0 ? p + (p >> SH) : p; }
   static __host__ __device__ constexpr int span(int n) { return pad(n - 1) + 1; }
   __device__ __forceinline__ float2& at(int p) const { return base[pad(p) * stride]; }
+  // element p + off where pad(p + off) = pad(p) + pad(off): off a multiple of
+  // Q, or p a multiple of Q.  With off a compile-time constant the address is
+  // one register plus an immediate offset.
+  static constexpr int Q = SH > 0 ? (1 << SH) : 1;
+  __device__ __forceinline__ float2& at(int p, int off) const { return base[(pad(p) + pad(off)) * stride]; }
   template <int B> __device__ __forceinline__ const LdsLine& get() const { return *this; }
 };
 
@@ -188,6 +193,7 @@ template <int SH> struct LdsLine {
 // two transforms of a pass (which run in opposite directions).
 template <class LL> struct DualLine {
   static constexpr bool kDual = true;
+  static constexpr int Q = LL::Q;
   LL a, b;
   template <int B> __device__ __forceinline__ const LL& get() const { return B ? b : a; }
 };
@@ -228,8 +234,12 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
       for (int m = 0; m < R; ++m) out[b + m * NB] = a[m];
     } else {
       const int base = (j / L) * L * R + k;
+      constexpr int Q = LL::Q;
 #pragma unroll
-      for (int m = 0; m < R; ++m) lds.template get<BUF>().at(base + m * L) = a[m];
+      for (int m = 0; m < R; ++m) {
+        if constexpr (L % Q == 0 || (L == 1 && R % Q == 0)) lds.template get<BUF>().at(base, m * L) = a[m];
+        else lds.template get<BUF>().at(base + m * L) = a[m];
+      }
     }
   }
   if constexpr (LAST) {
@@ -238,7 +248,10 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[N / T], int t, const 
   } else {
     S::sync();
 #pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = lds.template get<BUF>().at(t + c * T);
+    for (int c = 0; c < E; ++c) {
+      if constexpr (T % LL::Q == 0) v[c] = lds.template get<BUF>().at(t, c * T);
+      else v[c] = lds.template get<BUF>().at(t + c * T);
+    }
     if constexpr (!LL::kDual) S::sync();
   }
 }
