@@ -310,6 +310,10 @@ def main():
     ap.add_argument("--probes", type=int, default=4096)
     ap.add_argument("--batch", type=int, default=0, help="probes per launch (0: the batch-width autotuner's choice)")
     ap.add_argument("--no-tune-batch", action="store_true", help="with --batch 0: all probes in one launch")
+    ap.add_argument("--graph", choices=("on", "off"), default="on",
+                    help="timed region: replay each chunk from a hipGraph (on) or launch eagerly (off)")
+    ap.add_argument("--profile-configs", type=int, default=2,
+                    help="configurations run after the timed region with per-kernel HIP-event timing")
     ap.add_argument("--per-probe", type=int, default=8, help="probes timed on the per-probe path (0: skip)")
     ap.add_argument("--cpu-probes", type=int, default=64)
     ap.add_argument("--cpu-slices", type=int, default=40)
@@ -367,18 +371,26 @@ def main():
         best, cost = sim.autotune_batch(px, py, dz, nslices=8, adf=(60.0, 200.0))
         tuned = {"best": best, "us_per_probe_slice": {str(k): round(v, 4) for k, v in cost.items()}}
         log(f"[bench] batch width autotuner: {tuned}")
-    # warm-up configurations use seeds distinct from the timed ones
-    sim.stem_batched(px, py, dz, nconfig=args.warmup, seed=S.SEED + 1000, sigma=0.08, batch=args.batch)
-    torch.cuda.synchronize()
-    sim.set_profiling(True)
+    # warm-up configurations use seeds distinct from the timed ones (and
+    # capture the chunk graph when it is on)
+    use_graph = args.graph == "on"
+    sim.stem_batched(px, py, dz, nconfig=args.warmup, seed=S.SEED + 1000, sigma=0.08, batch=args.batch,
+                     use_graph=use_graph)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = sim.stem_batched(px, py, dz, nconfig=args.steps, seed=S.SEED, sigma=0.08, adf=(60.0, 200.0),
-                         batch=args.batch)
+                         batch=args.batch, use_graph=use_graph)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    ks = sim.kernel_stats()
     tm = sim.last_timings()
+    # per-kernel durations for the roofline: HIP events around each launch on
+    # the handle's stream (eager launches), after the timed region so the
+    # events cost it nothing
+    sim.set_profiling(True)
+    sim.stem_batched(px, py, dz, nconfig=max(1, args.profile_configs), seed=S.SEED + 2000, sigma=0.08,
+                     batch=args.batch)
+    torch.cuda.synchronize()
+    ks = sim.kernel_stats()
     sim.set_profiling(False)
     ps_per_s = args.steps * nprobe * nslices / dt
     adf = r["adf"]
@@ -408,6 +420,8 @@ def main():
             "traffic_source": "profiles/traffic_stem512.json (rocprofv3 FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, "
                               "per wave, measured on launches of the same width)" if traffic else None,
             "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": ks[kern]["ms_avg"],
+            "launch_timing": f"HIP events around each launch on the handle's stream, {ks[kern]['launches']} "
+                             f"eager launches over {max(1, args.profile_configs)} configurations after the timed region",
             "other_passes": {k: ks[k]["ms_avg"] for k in ("row", "column", "detector", "potential")}}
 
     per_probe = None
@@ -474,7 +488,7 @@ def main():
         "data": "synthetic (SrTiO3 supercell, seeded Philox frozen-phonon displacements)",
         "config": {"workload": "batched STEM + TDS, SrTiO3 10x10x50, 512^2 over 40 A, 300 kV, 2 A slices",
                    "probes": nprobe, "slices": nslices, "configs_per_step": 1, "batch": B,
-                   "batch_autotuner": tuned,
+                   "batch_autotuner": tuned, "graph_replay": use_graph,
                    "adf_mrad": [60, 200], "step": "one frozen-phonon configuration"},
         "stage_ms_total": tm,
         "per_probe_path": per_probe,
