@@ -86,8 +86,10 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
       gu[c] = ok ? gu[c] : zero;
       w[c] = ok ? w[c] : zero;
     }
-    fft_line<Plan<N>, N, +1, S>(gu, t, stw, L);              // G_u = IF_x(G_a)
-    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S>(w, t, stw, L);  // w = IF_x(z_{i-1})
+    // band-limited inputs (PRUNE): the first butterflies skip the zero blocks
+    constexpr bool ZIN = PRUNE && E % 4 == 0 && Plan<N>::R1 % 4 == 0;
+    fft_line<Plan<N>, N, +1, S, LL, ZIN>(gu, t, stw, L);              // G_u = IF_x(G_a)
+    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S, LL, ZIN>(w, t, stw, L);  // w = IF_x(z_{i-1})
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       const float2 gw = cmulc(gu[c], tr[c]);     // conj(t) G_u
