@@ -166,18 +166,28 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
   using LL = LdsLine<Cfg::SH>;
   constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
-  const long long line = blk * LINES + ln;
-  const bool ok = line < (long long)a.nbatch * a.ny;
-  const int b = ok ? (int)(line / a.ny) : 0;
-  const int y = ok ? (int)(line - (long long)b * a.ny) : 0;
+  // wave b0 and row of the block's first line on the scalar unit (blk is
+  // uniform and inside the segment, so b0 < nbatch); the thread's line is
+  // db waves further (as in k_row)
+  const long long first = blk * LINES;
+  const long long b0 = first / a.ny;
+  int y = (int)(first - b0 * a.ny) + ln;
+  int db = 0;
+  while (y >= a.ny) {  // at most LINES / ny + 1 steps; none when ny % LINES == 0
+    y -= a.ny;
+    ++db;
+  }
+  const bool ok = b0 + db < a.nbatch;
+  const int b = ok ? (int)b0 + db : 0;
+  auto roff = [&](long long bs) { return b0 * bs + ((long long)db * bs + (long long)y * N); };
   float2 v[E];
-  {
-    const float2* row = a.src + (long long)b * a.src_bstride + (long long)y * N;
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const bool load = ok && (!(FL & PRUNE_IN) || BandBlocks<E>::active(c));
-      v[c] = load ? row[t + c * T] : make_float2(0.f, 0.f);
-    }
+  for (int c = 0; c < E; ++c) v[c] = make_float2(0.f, 0.f);
+  if (ok) {
+    const float2* row = a.src + roff(a.src_bstride);
+#pragma unroll
+    for (int c = 0; c < E; ++c)
+      if (!(FL & PRUNE_IN) || BandBlocks<E>::active(c)) v[c] = row[t + c * T];
   }
   const LL L{lds + ln * LS, 1};
   constexpr bool ZIN = (FL & PRUNE_IN) != 0 && E % 4 == 0 && P::R1 % 4 == 0;  // first transform's input
@@ -185,14 +195,15 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
   if constexpr ((FL & STORE_AUX) != 0) {
     // read back only much later (by the adjoint pass): streamed past the caches
     if (ok) {
-      float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
+      float2* row = a.aux + roff(a.dst_bstride);
 #pragma unroll
       for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
     }
   }
   if constexpr (MUL == RM_T) {
     if (ok) {
-      const long long slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
+      long long slot = a.t_index ? (long long)a.t_index[b0] : b0 / a.t_group;
+      if (db != 0) slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
 #pragma unroll
       for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
@@ -222,7 +233,7 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
   if constexpr (MODE == ROW_FWD) fft_line<P, N, -1, S, LL, ZIN>(v, t, stw, L);
   if constexpr (MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
   if (ok) {
-    float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
+    float2* row = a.dst + roff(a.dst_bstride);
 #pragma unroll
     for (int c = 0; c < E; ++c)
       if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];
