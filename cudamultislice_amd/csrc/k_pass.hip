@@ -942,6 +942,8 @@ hipError_t launch_row_group(int mode, int mul, const PassGroup<RowArgs>& h, cons
   if (!d) return hipErrorInvalidValue;
   const int fl = (h.a[0].prune_in ? PRUNE_IN : 0) | (h.a[0].prune_out ? PRUNE_OUT : 0);
   if (total > 0x7fffffffLL) return hipErrorInvalidValue;
+  for (int k = 0; k < h.nseg; ++k)
+    if (h.a[k].ny <= 0) return hipErrorInvalidValue;  // the kernels walk rows in steps of ny
   const GroupHdr hd = group_hdr(h);
 #define MSL_RG(M, U, F) hipLaunchKernelGGL((k_row_group<M, U, F>), dim3((unsigned)total), dim3(256), 0, s, d, hd)
   if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_RG(ROW_INV_MUL_FWD, RM_T, PRUNE_OUT);
@@ -1116,6 +1118,8 @@ static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int 
 }
 
 hipError_t launch_row_v(int mode, int mul, const RowArgs& a, hipStream_t s, int var) {
+  // the kernels walk rows in steps of ny: a grid needs at least one row
+  if (a.ny <= 0 || a.nbatch < 0) return hipErrorInvalidValue;
   switch (a.nx) {
     case 16: return row_n<16>(mode, mul, a, s, var);
     case 32: return row_n<32>(mode, mul, a, s, var);
