@@ -73,6 +73,7 @@ _SIGS = {
     "msl_multislice_step_rs": ([_vp, _i], _i),
     "msl_get_slices": ([_vp, _i, C.POINTER(_i)], _i),
     "msl_get_exit_wave": ([_vp, _vp], _i), "msl_get_stem_exit_wave": ([_vp, _vp], _i),
+    "msl_get_diffraction_display": ([_vp, _vp], _i),
     "msl_alloc_tds": ([_vp], _i), "msl_add_tds_waves": ([_vp], _i), "msl_get_tds": ([_vp, _vp], _i),
     "msl_simulate_image": ([_vp, _f], _i), "msl_get_image": ([_vp, _vp], _i),
     "msl_ctf_sweep": ([_vp, _i, _vp, _vp], _i),

@@ -855,6 +855,29 @@ msl_status msl_get_stem_exit_wave(msl_sim* s, float* out) {
   API_END(s)
 }
 
+// Diffraction log display of the centred spectrum C that GetSTEMExitWave
+// reads back (the diffpattern branch of conjugationKernel, U.cu:230-231, the
+// mode U.cu:2198 marks "Should be 1"): for pixel (x, y), with (x3, y3) the
+// half-grid shift of (x, y) and (x4, y4) = (nx-1-x3, ny-1-y3),
+// out = log(1 + 3000 |C[y3][x3] + conj(C[y4][x4])|^2), in float.
+msl_status msl_get_diffraction_display(msl_sim* s, float* out) {
+  API_BEGIN(s)
+  require(out != nullptr, MSL_ERR_ARG, "null output");
+  require(s->have_waves, MSL_ERR_STATE, "no wave");
+  const int nx = s->resX, ny = s->resY;
+  std::vector<float2> C((size_t)nx * ny);
+  const msl_status st = msl_get_stem_exit_wave(s, reinterpret_cast<float*>(C.data()));
+  if (st != MSL_OK) return st;
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const int x3 = x < nx / 2 ? x + nx / 2 : x - nx / 2, y3 = y < ny / 2 ? y + ny / 2 : y - ny / 2;
+      const float2 a = C[(size_t)y3 * nx + x3], b = C[(size_t)(ny - 1 - y3) * nx + (nx - 1 - x3)];
+      const float re = a.x + b.x, im = a.y - b.y;
+      out[(size_t)y * nx + x] = std::log(1.0f + 3000.0f * (re * re + im * im));
+    }
+  API_END(s)
+}
+
 msl_status msl_alloc_tds(msl_sim* s) {
   API_BEGIN(s)
   require(s->have_freqs, MSL_ERR_STATE, "PreCalculateFrequencies first");

@@ -133,6 +133,13 @@ class MultisliceSimulation:
         self._c(lib().msl_get_stem_exit_wave(self._h, _p(o)))
         return o
 
+    def diffraction_display(self):
+        """log(1 + 3000 |.|^2) display of GetSTEMExitWave's centred spectrum
+        (U.cu:230-231)"""
+        o = np.zeros((self.resY, self.resX), np.float32)
+        self._c(lib().msl_get_diffraction_display(self._h, _p(o)))
+        return o
+
     def AllocTDS(self):
         self._c(lib().msl_alloc_tds(self._h))
 

@@ -123,6 +123,12 @@ msl_status msl_get_slices(const msl_sim* sim, int conventional, int* out);
 msl_status msl_get_exit_wave(msl_sim* sim, float* out);
 /* GetSTEMExitWave, U.cu:2180-2229: fftshifted FFT of the wave */
 msl_status msl_get_stem_exit_wave(msl_sim* sim, float* out);
+/* diffraction log display of that centred spectrum C (the diffpattern branch
+ * of conjugationKernel, U.cu:230-231, the mode U.cu:2198 marks "Should be
+ * 1"): out[y][x] = log(1 + 3000 |C[y3][x3] + conj(C[ny-1-y3][nx-1-x3])|^2),
+ * (x3, y3) the half-grid shift of (x, y); out is resX*resY floats; also sets
+ * maxposEW / minposEW as msl_get_stem_exit_wave */
+msl_status msl_get_diffraction_display(msl_sim* sim, float* out);
 /* AllocTDS / AddTDSWaves / GetTDS*, U.cu:2726-2791 */
 msl_status msl_alloc_tds(msl_sim* sim);
 msl_status msl_add_tds_waves(msl_sim* sim);
@@ -188,7 +194,9 @@ typedef struct {
                              configuration; ignored with exit_out or while per-kernel
                              profiling is on */
   float* adf_out;         /* [nprobe], mean over configurations */
-  float* pacbed_out;      /* optional [resY][resX]: mean |Psi_hat|^2 (unshifted) */
+  float* pacbed_out;      /* optional [resY][resX]: mean over probes and configurations of
+                             |Psi_hat|^2, Psi_hat the unnormalised FFT of the exit wave
+                             (unshifted; sums to resX*resY times the mean intensity) */
   float* exit_out;        /* optional [nconfig][nprobe][resY][resX][2] exit waves (small runs) */
 } msl_stem_batch_desc;
 msl_status msl_stem_batched(msl_sim* sim, const msl_stem_batch_desc* desc);

@@ -314,6 +314,14 @@ void orc_init_stem(void* p, int px, int py, float dz) {
 }
 void orc_step_conv(void* p, int it, int potmode) { dispatch((Handle*)p, [&](auto& s) { s.step_conv(it, potmode); return 0; }); }
 void orc_step_fd(void* p, int it) { dispatch((Handle*)p, [&](auto& s) { s.step_fd(it); return 0; }); }
+// a fixed real potential for step_fd (NULL restores the atoms' potential)
+void orc_set_fd_potential(void* p, const double* V) {
+  dispatch((Handle*)p, [&](auto& s) {
+    if (!V) s.fd_potential_override.clear();
+    else s.fd_potential_override.assign(V, V + s.npix());
+    return 0;
+  });
+}
 void orc_step_rs(void* p, int it) { dispatch((Handle*)p, [&](auto& s) { s.step_rs(it); return 0; }); }
 // the real-space second-derivative stencil of MultisliceStepRS on a wave
 void orc_rs_laplacian(void* p, const double* in, double* out) {

@@ -798,10 +798,19 @@ template <class Real> struct Sim {
   // MultisliceStep (finite difference), U.cu:2040-2083 with the kernels
   // BandLimitKernel (1025-1042), gradKernel (948-962) and
   // finiteDifferenceKernel (995-1023).
+  // Test hook: a caller-given real potential replaces the atoms' one in
+  // step_fd (empty: the potential of U.cu:883-946).
+  std::vector<Real> fd_potential_override;
+
   void step_fd(int it) {
     Real z = Real(slice_z(dzFD, it));
     std::vector<C> V;
-    potential_fd(z, V);
+    if (fd_potential_override.size() == npix()) {
+      V.resize(npix());
+      for (size_t i = 0; i < npix(); ++i) V[i] = C(fd_potential_override[i], 0);
+    } else {
+      potential_fd(z, V);
+    }
     fft.run(V.data(), false);
     bandlimit(V);
     fft.run(V.data(), true);

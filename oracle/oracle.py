@@ -49,6 +49,7 @@ def _lib():
         "orc_init_plane": ([vp, f], None), "orc_init_plane_fd": ([vp], None),
         "orc_init_stem": ([vp, i, i, f], None),
         "orc_step_conv": ([vp, i, i], None), "orc_step_fd": ([vp, i], None), "orc_step_rs": ([vp, i], None),
+        "orc_set_fd_potential": ([vp, vp], None),
         "orc_rs_laplacian": ([vp, _dp, _dp], None),
         "orc_probe": ([vp, i, i, _dp], None),
         "orc_get_wave": ([vp, i, _dp], None), "orc_set_wave": ([vp, i, _dp], None),
@@ -238,6 +239,14 @@ class OracleSim:
     def step_conv(self, it, potmode=0): L().orc_step_conv(self._h, it, potmode)
     def step_fd(self, it): L().orc_step_fd(self._h, it)
     def step_rs(self, it): L().orc_step_rs(self._h, it)
+
+    def set_fd_potential(self, V=None):
+        """fixed real potential [resY][resX] for step_fd; None: the atoms'"""
+        if V is None:
+            L().orc_set_fd_potential(self._h, None)
+        else:
+            self._fdV = np.ascontiguousarray(V, np.float64)
+            L().orc_set_fd_potential(self._h, self._fdV.ctypes.data_as(C.c_void_p))
 
     def rs_laplacian(self, w):
         w = np.ascontiguousarray(w, np.complex128)

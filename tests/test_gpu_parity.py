@@ -287,6 +287,37 @@ def test_stem_probe_steps_and_tds_match_oracle(fp):
     assert rel_l2(g.GetTDS(), o.tds()) < 1e-4
 
 
+def test_stem_exit_wave_readback_and_diffraction_display(fp):
+    """GetSTEMExitWave (U.cu:2180-2229): the unnormalised spectrum of the exit
+    wave, centred by a half-grid shift, with maxposEW / minposEW the largest
+    and smallest moduli; the diffraction log display of that spectrum
+    (U.cu:230-231) equals log(1 + 3000 |F(k) + conj(F(-k-1))|^2) on the
+    uncentred spectrum F (the display's two index maps composed)."""
+    Z, x, y, z = _si_small()
+    g, o = _both(Z, x, y, z, fp)
+    _setup(g, o, 300.0, 0.25, 64, 20.0, stem=True, obj=25.0)
+    g.UploadBinnedAtoms()
+    o.bin()
+    g.STEMInitialiseWavefunctions(20, 33)
+    g.set_conventional_dz(2.0)
+    o.init_stem(20, 33, 2.0)
+    for it in range(1, 6):
+        g.MultisliceStepConv(it)
+        o.step_conv(it)
+    F = O.fft2(o.wave())
+    centred = np.roll(F, (32, 32), axis=(0, 1))
+    got = g.GetSTEMExitWave()
+    assert rel_l2(got, centred) < 1e-4
+    inf = g.info()
+    assert inf["maxposEW"] == pytest.approx(np.abs(centred).max(), rel=1e-4)
+    assert inf["minposEW"] <= np.abs(centred).max() * 1e-3
+    disp = g.diffraction_display()
+    mirror = F[(-1 - np.arange(64)) % 64][:, (-1 - np.arange(64)) % 64]
+    want = np.log1p(3000.0 * np.abs(F + np.conj(mirror)) ** 2)
+    assert np.max(np.abs(disp - want)) < 1e-4 * np.max(want)
+    assert disp.min() >= 0 and disp.max() > 1.0
+
+
 def test_ctf_image_matches_oracle(fp, tmp_path):
     Z, x, y, z = _si_small()
     g, o = _both(Z, x, y, z, fp)

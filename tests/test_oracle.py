@@ -302,9 +302,9 @@ def test_potential_linear_in_atoms(fp):
         assert ab.max() > 0
 
 
-def test_projected_mode_equals_integrated_for_thin_atoms(fp):
-    """Projected mode on the xy distance vs the 20-sample z integral of the
-    3D potential: close when the slab holds the whole atom profile."""
+def test_projected_mode_counts_only_the_slices_own_slabs(fp):
+    """The projected mode puts an atom's whole projection into the slice
+    whose slabs hold it, and nothing into the neighbouring slice."""
     Z = np.array([38, 6, 6], np.int32)
     x = np.array([5.0, -30.0, 40.0], np.float32)
     y = np.array([5.0, -30.0, 40.0], np.float32)
@@ -685,3 +685,286 @@ def test_rs_step_in_vacuum_follows_the_series_closed_form(fp):
         assert np.allclose(s.wave(0), w, rtol=1e-12, atol=0)
         assert np.allclose(s.wave(2), w, rtol=1e-12, atol=0)
     assert np.allclose(s.wave(1), w * w, rtol=1e-12)  # step 2: PsiMinus = PsiPlus * Psi
+
+
+# ---------------------------------------------------------------- further pins
+def _brute_fd(s, Z, x, y, z, zc, N, ps):
+    """binnedAtomPKernelFD (U.cu:883-946) without bins: pixel (i ps, j ps)
+    (no block offset), the atoms of 1 A slabs floor(H - z) +- 3, the 3D
+    r^2 < 5 test taken in FP32 as the kernel takes it, vatom summed at the
+    plane z."""
+    e = s.extents()
+    H = np.float32(e[4] - e[5])
+    kc = int(np.floor(np.float32(H - np.float32(zc)) / np.float32(1)))
+    zid = np.floor((np.float32(e[4]) - z) / np.float32(1)).astype(int)
+    X, Y, Zr = (x - np.float32(e[1])), (y - np.float32(e[3])), (z - np.float32(e[5]))
+    ps32 = np.float32(ps)
+    out = np.zeros((N, N))
+    for iy in range(N):
+        for ix in range(N):
+            pxf, pyf = np.float32(np.float32(ix) * ps32), np.float32(np.float32(iy) * ps32)
+            acc = 0.0
+            for a in range(len(Z)):
+                if abs(zid[a] - kc) > 3:
+                    continue
+                dxf, dyf = np.float32(pxf - X[a]), np.float32(pyf - Y[a])
+                ddf = np.float32(np.float32(zc) - Zr[a])
+                if not _f32_fma(ddf, ddf, _f32_fma(dyf, dyf, np.float32(dxf * dxf))) < np.float32(5):
+                    continue
+                r2 = (float(pxf) - float(X[a])) ** 2 + (float(pyf) - float(Y[a])) ** 2 + (zc - float(Zr[a])) ** 2
+                acc += O.vatom(int(Z[a]), r2, s._fp)
+            out[iy, ix] = acc
+    return out
+
+
+def test_fd_potential_brute_force(fp):
+    """The finite-difference potential (mode 2) against a bin-free sum over
+    every atom: the same values wherever the slab window and cutoff admit
+    an atom, zero elsewhere."""
+    Z = np.array([14, 38, 8, 6, 6], np.int32)
+    x = np.array([1.0, 2.1, 1.4, -2.0, 40.0], np.float32)
+    y = np.array([1.2, 0.9, 2.3, -2.0, 40.0], np.float32)
+    z = np.array([3.0, 4.2, 2.1, 0.0, 9.0], np.float32)
+    s = _sim_with(Z, x, y, z, fp)
+    s._fp = fp
+    N, ps = 14, 0.5
+    s.apply_microscope(200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1)
+    e = s.extents()
+    H = float(e[4] - e[5])
+    for depth in (6.3, 7.05, 9.5):
+        zc = float(np.float32(H - depth))
+        got = s.potential(2, zc, 1.0)
+        assert np.all(got.imag == 0)
+        want = _brute_fd(s, Z, x, y, z, zc, N, ps)
+        assert np.max(want) > 1.0
+        assert np.max(np.abs(got.real - want)) <= 1e-12 * np.max(want)
+        assert np.all((got.real == 0) == (want == 0))
+
+
+def _fd_dz_reference_formula(kV, ps, N):
+    """SetCalculationVariables' step (U.cu:1791-1834), restated in float64
+    from the published constants: the root X of A X^2 + B X + 3 = 0 and
+    0.99 sqrt(X)."""
+    Pi, e, m, c2 = 3.1415926, 1.6e-19, 9.11e-31, 9e16
+    eV = e * kV * 1e3
+    lam = 6.63e-34 * 3e8 / np.sqrt(eV * (2 * m * c2 + eV)) * 1e10
+    sig = 2 * Pi / (lam * kV * 1e3) * (m * c2 + eV) / (2 * m * c2 + eV)
+    V2 = kV * 1e3
+    p = N / (2 * ps * N)
+    ke2 = 0.666666 * 2 * p * p
+    u = ke2 - sig * V2 / (Pi * lam)
+    A = 16 * Pi ** 4 * u * u
+    B = 16 * Pi ** 2 * (u - 1 / (4 * lam * lam))
+    X = (-B + np.sqrt(B * B - 12 * A)) / (2 * A)
+    return 0.99 * np.sqrt(X), lam, sig
+
+
+@pytest.mark.parametrize("kV,ps,N", [(300.0, 0.1, 128), (200.0, 0.2, 64), (300.0, 0.02, 128), (100.0, 0.05, 256)])
+def test_fd_step_size_is_stable_and_follows_the_quadratic(fp, kV, ps, N):
+    """dzFD (U.cu:1815-1834): equal to an independent float64 evaluation of
+    the reference's quadratic (FP32 rounding only), and inside the von
+    Neumann stability region of the scheme it drives.  For a mode of
+    transverse k^2 in a potential V the update c+ r^2 - b r + c- = 0, with
+    c+- = 1 +- 2 pi i dz / lambda and b = 2 + 4 pi^2 dz^2 (k^2 - sigma V /
+    (pi lambda)), keeps both roots on the unit circle iff
+    b^2 <= 4 |c+|^2.  At the step size this holds for every k^2 up to the
+    band edge k_e^2 = 2/3 (kx_N^2 + ky_N^2) and V from 0 to the accelerating
+    voltage (the formula's design range), and the step is a fixed fraction
+    (0.70-0.75) of the stability limit at k_e^2, V = kV."""
+    s = _sim_with(*S.silicon(1, 1, 1), fp)
+    s.apply_microscope(kV)
+    s.set_calc(ps, 0.0, N, N, 10, 10, 20.0)
+    sc = s.scalars()
+    want, lam, sig = _fd_dz_reference_formula(kV, ps, N)
+    assert sc["dzFD"] == pytest.approx(want, rel=2e-5)
+    dz = sc["dzFD"]
+    ke2 = 0.666666 * 2 * (1 / (2 * ps)) ** 2
+    for V in np.linspace(0.0, kV * 1e3, 7):
+        for k2 in np.linspace(0.0, ke2, 9):
+            u = k2 - sig * V / (np.pi * lam)
+            b = 2 + 4 * np.pi ** 2 * dz ** 2 * u
+            assert b * b <= 4 * (1 + (2 * np.pi * dz / lam) ** 2), (V, k2)
+            r = np.roots([1 + 2j * np.pi * dz / lam, -b, 1 - 2j * np.pi * dz / lam])
+            assert np.allclose(np.abs(r), 1.0, atol=1e-6)
+    u = ke2 - sig * kV * 1e3 / (np.pi * lam)
+    limit = np.sqrt((1 / lam ** 2 - u) / (np.pi ** 2 * u * u))
+    assert 0.70 < dz / limit < 0.75, dz / limit
+
+
+def test_fd_dispersion_in_a_constant_potential(fp):
+    """FD scheme (U.cu:995-1023) on a tilted plane wave in a constant
+    potential V0: the characteristic equation gains the coupling term,
+    c+ r^2 - (2 + 4 pi^2 k^2 dz^2 - 4 pi sigma V0 dz^2 / lambda) r + c- = 0
+    (pi = 3.14159 as the kernel writes it); a constant potential survives
+    the band limit unchanged."""
+    N, ps = 32, 0.3
+    s = _vacuum_sim(fp, N, ps, kV=200.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 0, 0)
+    sc = s.scalars()
+    dz, lam, sig = float(np.float32(sc["dzFD"])), sc["wavel"], sc["sigma2"]
+    kx, _ = s.freqs()
+    j = 4
+    k = float(kx[j])
+    p9 = 3.14159
+    for V0 in (15.0, 120.0):
+        b = 2 + 4 * p9 ** 2 * k * k * dz ** 2 - 4 * p9 * sig * V0 * dz ** 2 / lam
+        a = 2 * p9 * dz / lam
+        roots = np.roots([1 + 1j * a, -b, 1 - 1j * a])
+        pw = np.tile(np.exp(2j * np.pi * j * np.arange(N) / N), (N, 1))
+        psi_m, psi0 = 1.0 + 0j, np.exp(-0.3j)
+        A = np.linalg.solve(np.array([[1, 1], roots]), np.array([psi_m, psi0]))
+        s.init_plane_fd()
+        s.set_fd_potential(np.full((N, N), V0))
+        s.set_wave(pw * psi_m, 1)
+        s.set_wave(pw * psi0, 0)
+        for n in range(1, 7):
+            s.step_fd(n)
+            want = A[0] * roots[0] ** (n + 1) + A[1] * roots[1] ** (n + 1)
+            assert np.max(np.abs(s.wave() - want * pw)) < 1e-10, (V0, n)
+        # the potential shifts the phase velocity: not the vacuum solution
+        rv = np.roots([1 + 1j * a, -(2 + 4 * p9 ** 2 * k * k * dz ** 2), 1 - 1j * a])
+        Av = np.linalg.solve(np.array([[1, 1], rv]), np.array([psi_m, psi0]))
+        assert np.max(np.abs(s.wave() - (Av[0] * rv[0] ** 7 + Av[1] * rv[1] ** 7) * pw)) > 1e-6
+    s.set_fd_potential(None)
+
+
+def test_integrated_mode_over_all_slices_is_the_truncated_projection(fp):
+    """Summed over the 1 A slices that cover an isolated atom, the
+    integrated mode (20 samples per slice, U.cu:582-653) is the z integral
+    of vatom over the sphere r^2 < 5 (Riemann sum of step dz/20), and the
+    projected mode (U.cu:730-795) the whole projection vzatom: the two
+    modes agree up to the tail beyond the cutoff, which a quadrature of
+    vatom gives independently."""
+    from scipy.integrate import quad
+    Z = np.array([38, 6, 6], np.int32)
+    x = np.array([5.0, -30.0, 40.0], np.float32)
+    y = np.array([5.0, -30.0, 40.0], np.float32)
+    z = np.array([10.5, 0.0, 30.0], np.float32)
+    s = _sim_with(Z, x, y, z, fp)
+    N, ps, off = 16, 0.25, 33.0
+    s.apply_microscope(300.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+    e = s.extents()
+    H = float(e[4] - e[5])
+    n = int(np.ceil(H))
+    integ = sum(s.potential(3, float(np.float32(H - it)), 1.0).real for it in range(1, n + 1))
+    proj = sum(s.potential(4, float(np.float32(H - it)), 1.0).real for it in range(1, n + 1))
+    X = float(np.float32(5.0) - e[1])
+    c = off + np.arange(N) * ps
+    checked = 0
+    for iy in range(N):
+        for ix in range(N):
+            r2 = (c[ix] - X) ** 2 + (c[iy] - X) ** 2
+            if not 0.25 < r2 < 2.0:
+                continue
+            zmax = np.sqrt(5.0 - r2)
+            full = O.vzatom(38, r2, fp)
+            cut = quad(lambda t: O.vatom(38, r2 + t * t, fp), -zmax, zmax, limit=200)[0]
+            assert proj[iy, ix] == pytest.approx(full, rel=1e-12)
+            # a right-end Riemann sum of step 0.05 A of a function cut off at
+            # its ends: within 1 % (one sample in 20 dropped would be 5 %)
+            assert integ[iy, ix] == pytest.approx(cut, rel=1e-2)
+            assert 0.8 * full < integ[iy, ix] < full
+            checked += 1
+    assert checked >= 12
+
+
+def _grating(N, j, l, eps):
+    yy, xx = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    return np.exp(1j * eps * np.cos(2 * np.pi * (j * xx + l * yy) / N)), np.cos(2 * np.pi * (j * xx + l * yy) / N)
+
+
+def test_ctf_spatial_envelope_closed_form(fp):
+    """Partial spatial coherence of imagingKernel2 (U.cu:1064-1066) against
+    the textbook envelope E_s(k) = exp(-(pi alpha / lambda)^2 (Cs lambda^3 k^3
+    + df lambda k)^2), alpha the illumination semi-angle (Kirkland, Advanced
+    Computing in Electron Microscopy, ch. 3): a weak phase grating at k0 images
+    as 1 + 2 eps E_s(k0) sin chi(k0) cos(2 pi k0 x)."""
+    N, ps, Cs, df, beta = 64, 0.25, 1.2e7, -500.0, 0.6
+    s = _vacuum_sim(fp, N, ps, kV=200.0)
+    s.apply_microscope(200.0, df=df, Cs=Cs, beta=beta, obj=1000.0)
+    s.set_calc(ps, df, N, N, 1, 1, 22, 62.0, 62.0)
+    s.init_plane(1.0)
+    lam = s.scalars()["wavel"]
+    eps = 1e-4
+    for j in (3, 7, 12, 17):
+        ph, cosg = _grating(N, j, 0, eps)
+        s.set_wave(ph)
+        k0 = j / (N * ps)
+        chi = np.pi * lam * k0 ** 2 * (0.5 * Cs * lam ** 2 * k0 ** 2 + df)
+        Es = np.exp(-(np.pi * beta * 1e-3 / lam) ** 2 * (Cs * lam ** 3 * k0 ** 3 + df * lam * k0) ** 2)
+        assert Es < 0.999 or j == 3
+        want = 1 + 2 * eps * Es * np.sin(chi) * cosg
+        assert np.max(np.abs(s.ctf_image(df) - want)) < 5e-8, j
+
+
+def test_ctf_temporal_envelope_is_a_defocus_average(fp):
+    """Partial temporal coherence of imagingKernel2 (U.cu:1067-1068): to first
+    order in a weak phase grating, the image with spread delta equals the
+    average of coherent images over Gaussian defocus offsets of standard
+    deviation delta / sqrt(2) (Gauss-Hermite quadrature of the oracle's own
+    delta = 0 images), independent of the envelope's formula."""
+    N, ps, Cs, df, delta = 64, 0.25, 1.2e7, -500.0, 60.0
+    s = _vacuum_sim(fp, N, ps, kV=200.0)
+    s.set_calc(ps, df, N, N, 1, 1, 22, 62.0, 62.0)
+    s.init_plane(1.0)
+    xs, ws = np.polynomial.hermite_e.hermegauss(40)
+    ws = ws / ws.sum()
+    eps = 1e-4
+    for j in (5, 11, 16):
+        ph, _ = _grating(N, j, 0, eps)
+        s.set_wave(ph)
+        s.apply_microscope(200.0, df=df, Cs=Cs, delta=delta, obj=1000.0)
+        I = s.ctf_image(df)
+        s.apply_microscope(200.0, df=df, Cs=Cs, obj=1000.0)
+        avg = sum(w * s.ctf_image(df + delta / np.sqrt(2) * xi) for xi, w in zip(xs, ws))
+        assert np.max(np.abs(I - avg)) < 5e-8, j
+        assert np.max(np.abs(I - s.ctf_image(df))) > 1e-6  # the spread acts at this k
+
+
+def test_ctf_twofold_astigmatism_phase(fp):
+    """Twofold astigmatism A1 in chi (U.cu:1070): chi = pi lambda k^2 A1
+    sin(2 (phi - phi_A1)), phi = atan2(ky, kx); weak phase gratings along
+    several directions image as 1 + 2 eps sin chi(k0) cos(2 pi k0 . r)."""
+    N, ps, A1, A1phi = 64, 0.25, 150.0, 0.3
+    s = _vacuum_sim(fp, N, ps, kV=200.0)
+    s.apply_microscope(200.0, A1=A1, A1phi=A1phi, obj=1000.0)
+    s.set_calc(ps, 0.0, N, N, 1, 1, 22, 62.0, 62.0)
+    s.init_plane(1.0)
+    lam = s.scalars()["wavel"]
+    eps = 1e-4
+    seen = []
+    for j, l in ((6, 0), (0, 6), (5, 5), (-4, 7), (9, -3)):
+        ph, cosg = _grating(N, j, l, eps)
+        s.set_wave(ph)
+        kx, ky = j / (N * ps), l / (N * ps)
+        chi = np.pi * lam * (kx * kx + ky * ky) * A1 * np.sin(2 * (np.arctan2(ky, kx) - A1phi))
+        seen.append(np.sin(chi))
+        want = 1 + 2 * eps * np.sin(chi) * cosg
+        assert np.max(np.abs(s.ctf_image(0.0) - want)) < 5e-8, (j, l)
+    assert max(seen) > 0.3 and min(seen) < -0.3  # both signs of the twofold term
+
+
+def test_aberrated_probe_phase(fp):
+    """STEM probe with Cs and defocus (U.cu:368-375): its spectrum inside
+    the aperture is a constant times exp(-i chi(k)), chi = pi lambda k^2
+    (Cs lambda^2 k^2 / 2 + df), and zero outside."""
+    N, ps = 64, 0.2
+    s = _vacuum_sim(fp, N, ps, kV=300.0)
+    for Cs, df in ((1e4, -50.0), (1.2e7, 300.0), (0.0, -700.0)):
+        s.apply_microscope(300.0, df=df, Cs=Cs, obj=25.0)
+        s.set_calc(ps, df, N, N, 1, 1, 22, 0, 0, stem=True)
+        lam = s.scalars()["wavel"]
+        p = s.probe(0, 0)
+        F = np.fft.fft2(p)
+        kx, ky = s.freqs()
+        k2 = kx[None, :].astype(np.float64) ** 2 + ky[:, None].astype(np.float64) ** 2
+        inside = np.sqrt(k2) < 25e-3 / s.scalars()["wavel_f"]
+        chi = np.pi * lam * k2 * (0.5 * Cs * lam ** 2 * k2 + df)
+        assert np.ptp(chi[inside]) > 2.0  # the aberration matters
+        rel = F[inside] / F[0, 0] * np.exp(1j * chi[inside])
+        # pi is 3.14159265 in the probe kernel: a phase error of 3.6e-9 chi
+        assert np.all(np.abs(rel - 1.0) <= 4e-9 * np.abs(chi[inside]) + 1e-10), (Cs, df)
+        assert np.max(np.abs(F[~inside])) < 1e-12 * np.abs(F[0, 0])
+        assert np.sum(np.abs(p)) == pytest.approx(1.0, rel=1e-12)
