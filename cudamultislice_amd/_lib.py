@@ -88,6 +88,7 @@ _SIGS = {
     "msl_diff_forward_only": ([_vp, _vp, _vp, _vp, _vp], _i),
     "msl_diff_backward": ([_vp, _vp, _vp, _vp], _i),
     "msl_diff_info": ([_vp, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f), C.POINTER(C.c_longlong)], _i),
+    "msl_diff_set_chunk": ([_vp, _i], _i),
     "msl_fft2_device": ([_i, _i, _i, _vp, _i, _vp], _i),
     "msl_set_pass_variant": ([_i, _i, _i], _i), "msl_get_pass_variant": ([_i, _i, C.POINTER(_i)], _i),
     "msl_bench_pass": ([_i, _i, _i, _i, _i, C.POINTER(_f), C.POINTER(_f)], _i),

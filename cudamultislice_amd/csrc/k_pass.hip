@@ -156,126 +156,13 @@ template <int E> struct BandBlocks {
   static __host__ __device__ constexpr bool active(int c) { return c < LO || c >= E - LO; }
 };
 
-// the LINES lines of workgroup-sized block blk of a row pass; lds holds
-// LINES * LS elements, tw the twiddles of length N
+// The row pass body: the LINES lines of workgroup-sized block blk; lds holds
+// LINES * LS elements, tw the N twiddles staged in LDS.  k_row (one length
+// per launch, arguments in kernel registers) and k_row_group (several
+// lengths per launch, arguments read from a table in device memory) both
+// expand this one body.
 template <int N, int MODE, int MUL, int FL, int RV, class A>
 __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds, const float2* stw) {
-  using Cfg = RowCfg<N, RV>;
-  using P = typename Cfg::P;
-  using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
-  using LL = LdsLine<Cfg::SH>;
-  constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS;
-  const int tid = threadIdx.x, t = tid % T, ln = tid / T;
-  // wave b0 and row of the block's first line on the scalar unit (blk is
-  // uniform and inside the segment, so b0 < nbatch); the thread's line is
-  // db waves further (as in k_row)
-  const long long first = blk * LINES;
-  const long long b0 = first / a.ny;
-  int y = (int)(first - b0 * a.ny) + ln;
-  int db = 0;
-  while (y >= a.ny) {  // at most LINES / ny + 1 steps; none when ny % LINES == 0
-    y -= a.ny;
-    ++db;
-  }
-  const bool ok = b0 + db < a.nbatch;
-  const int b = ok ? (int)b0 + db : 0;
-  auto roff = [&](long long bs) { return b0 * bs + ((long long)db * bs + (long long)y * N); };
-  float2 v[E];
-#pragma unroll
-  for (int c = 0; c < E; ++c) v[c] = make_float2(0.f, 0.f);
-  if (ok) {
-    const float2* row = a.src + roff(a.src_bstride);
-#pragma unroll
-    for (int c = 0; c < E; ++c)
-      if (!(FL & PRUNE_IN) || BandBlocks<E>::active(c)) v[c] = row[t + c * T];
-  }
-  const LL L{lds + ln * LS, 1};
-  constexpr bool ZIN = (FL & PRUNE_IN) != 0 && E % 4 == 0 && P::R1 % 4 == 0;  // first transform's input
-  if constexpr (MODE == ROW_INV || MODE == ROW_INV_MUL_FWD) fft_line<P, N, +1, S, LL, ZIN>(v, t, stw, L);
-  if constexpr ((FL & STORE_AUX) != 0) {
-    // read back only much later (by the adjoint pass): streamed past the caches
-    if (ok) {
-      float2* row = a.aux + roff(a.dst_bstride);
-#pragma unroll
-      for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
-    }
-  }
-  if constexpr (MUL == RM_T) {
-    if (ok) {
-      long long slot = a.t_index ? (long long)a.t_index[b0] : b0 / a.t_group;
-      if (db != 0) slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
-      const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
-#pragma unroll
-      for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
-    }
-  } else if constexpr (MUL == RM_PHASE) {
-    if (ok) {
-      const float* vr = a.V + (long long)(b / a.t_group) * a.t_bstride + (long long)y * N;
-      // every potential value loaded before the first sincosf: its range
-      // reduction branches would otherwise put one memory round trip
-      // between consecutive elements
-      float ph[E];
-#pragma unroll
-      for (int c = 0; c < E; ++c) ph[c] = vr[t + c * T];
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        float sn, cs;
-        sincosf(a.sigma * ph[c], &sn, &cs);
-        v[c] = cscale(cmul(v[c], make_float2(cs, sn)), a.scale);
-      }
-    }
-  } else {
-    if (a.scale != 1.0f) {
-#pragma unroll
-      for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
-    }
-  }
-  if constexpr (MODE == ROW_FWD) fft_line<P, N, -1, S, LL, ZIN>(v, t, stw, L);
-  if constexpr (MODE == ROW_INV_MUL_FWD) fft_line<P, N, -1, S>(v, t, stw, L);
-  if (ok) {
-    float2* row = a.dst + roff(a.dst_bstride);
-#pragma unroll
-    for (int c = 0; c < E; ++c)
-      if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];
-  }
-}
-
-// Logical block (wave b, row block yb) = b ybs + yb of hardware workgroup h.
-// Each XCD takes a contiguous range of logical blocks (xcd_remap), and inside
-// it the same row block of kRowWaves consecutive waves runs back to back:
-// those waves read the same transmission rows, which then come from the L2
-// once instead of once per wave (a 2048^2 slice, 32 MiB, does not stay in an
-// XCD's L2 between one wave and the next).
-constexpr int kRowWaves = 8;
-__device__ __forceinline__ long long row_interleave(long long r, int nbatch, int ybs) {
-  const long long span = (long long)kRowWaves * ybs;
-  const long long full = (long long)(nbatch / kRowWaves) * span;  // blocks of complete wave groups
-  if (r >= full) return r;
-  const long long grp = r / span, rem = r - grp * span;
-  const int yb = (int)(rem / kRowWaves), bi = (int)(rem - (long long)yb * kRowWaves);
-  return (grp * kRowWaves + bi) * ybs + yb;
-}
-__device__ __forceinline__ long long row_order(long long h, long long total, int nbatch, int ybs) {
-  return row_interleave(xcd_remap(h, total), nbatch, ybs);
-}
-
-// The kernel keeps its own copy of the row_block body, on its own LDS arrays:
-// reached through pointer arguments the same body is scheduled differently
-// (fewer registers, less overlap) and ran 11 % slower at 512 on MI355X.
-template <int N, int MODE, int MUL, int FL, int RV>
-__global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_row(RowArgs a) {
-  __shared__ float2 lds[RowCfg<N, RV>::LINES * RowCfg<N, RV>::LS];
-  __shared__ float2 stw[N];  // twiddles staged in LDS: no global loads inside the transforms
-  for (int i = threadIdx.x; i < N; i += RowCfg<N, RV>::LINES * RowCfg<N, RV>::T) stw[i] = a.tw[i];
-  __syncthreads();
-  long long blk = blockIdx.x;
-  // wave-interleaved, XCD-aware order where a shared slice outgrows the L2
-  // (measured slower than the plain order at 1024 and below, where the slice
-  // stays in the L2 anyway)
-  if constexpr (N >= 2048 && MUL != RM_NONE)
-    if (N % RowCfg<N, RV>::LINES == 0 && a.ny == N)
-      blk = row_order(blockIdx.x, gridDim.x, a.nbatch, N / RowCfg<N, RV>::LINES);
-
   using Cfg = RowCfg<N, RV>;
   using P = typename Cfg::P;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
@@ -347,8 +234,8 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
         for (int q = 0; q < Q; ++q) {
           const float2 p = p0[q], g = v[c0 + q];
           const float w = dz * dz * 4.f * 3.14159f * a.sigma * vv[q] / wavel;
-          const float2 A = make_float2(2.f * p.x - dz * dz * g.x - w * p.x, 2.f * p.y - dz * dz * g.y - w * p.y);
-          v[c0 + q] = csub(cmul(rcp, A), cmul(ratio, pm[q]));
+          const float2 u = make_float2(2.f * p.x - dz * dz * g.x - w * p.x, 2.f * p.y - dz * dz * g.y - w * p.y);
+          v[c0 + q] = csub(cmul(rcp, u), cmul(ratio, pm[q]));
         }
         asm volatile("" ::: "memory");
       }
@@ -392,6 +279,41 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
     for (int c = 0; c < E; ++c)
       if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];
   }
+}
+
+// Logical block (wave b, row block yb) = b ybs + yb of hardware workgroup h.
+// Each XCD takes a contiguous range of logical blocks (xcd_remap), and inside
+// it the same row block of kRowWaves consecutive waves runs back to back:
+// those waves read the same transmission rows, which then come from the L2
+// once instead of once per wave (a 2048^2 slice, 32 MiB, does not stay in an
+// XCD's L2 between one wave and the next).
+constexpr int kRowWaves = 8;
+__device__ __forceinline__ long long row_interleave(long long r, int nbatch, int ybs) {
+  const long long span = (long long)kRowWaves * ybs;
+  const long long full = (long long)(nbatch / kRowWaves) * span;  // blocks of complete wave groups
+  if (r >= full) return r;
+  const long long grp = r / span, rem = r - grp * span;
+  const int yb = (int)(rem / kRowWaves), bi = (int)(rem - (long long)yb * kRowWaves);
+  return (grp * kRowWaves + bi) * ybs + yb;
+}
+__device__ __forceinline__ long long row_order(long long h, long long total, int nbatch, int ybs) {
+  return row_interleave(xcd_remap(h, total), nbatch, ybs);
+}
+
+template <int N, int MODE, int MUL, int FL, int RV>
+__global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_row(RowArgs a) {
+  __shared__ float2 lds[RowCfg<N, RV>::LINES * RowCfg<N, RV>::LS];
+  __shared__ float2 stw[N];  // twiddles staged in LDS: no global loads inside the transforms
+  for (int i = threadIdx.x; i < N; i += RowCfg<N, RV>::LINES * RowCfg<N, RV>::T) stw[i] = a.tw[i];
+  __syncthreads();
+  long long blk = blockIdx.x;
+  // wave-interleaved, XCD-aware order where a shared slice outgrows the L2
+  // (measured slower than the plain order at 1024 and below, where the slice
+  // stays in the L2 anyway)
+  if constexpr (N >= 2048 && MUL != RM_NONE)
+    if (N % RowCfg<N, RV>::LINES == 0 && a.ny == N)
+      blk = row_order(blockIdx.x, gridDim.x, a.nbatch, N / RowCfg<N, RV>::LINES);
+  row_block<N, MODE, MUL, FL, RV>(a, blk, lds, stw);
 }
 
 // ------------------------------------------------------------------ columns
@@ -440,7 +362,7 @@ template <int MUL, class A>
 __device__ __forceinline__ float2 kmul(const A& a, int x, int r, float kxv, float kyv, int2 pr) {
   if constexpr (MUL == CM_SCALE) {
     return make_float2(a.scale, 0.f);
-  } else if constexpr (MUL == CM_PROP || MUL == CM_PROPCONJ) {
+  } else if constexpr (MUL == CM_PROP || MUL == CM_PROPCONJ || MUL == CM_PROP_ZB || MUL == CM_PROPCONJ_ZB) {
     // not reached: col_body applies the propagator itself
     return make_float2(0.f, 0.f);
   } else if constexpr (MUL == CM_BAND) {
@@ -473,7 +395,11 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, co
   // frequencies are read only by modes that need them (scale-only passes may
   // run without k-grids)
   constexpr bool NEEDK = (Tr::MULT && (MUL == CM_BAND || MUL == CM_GRAD || MUL == CM_BANDGRAD));
-  constexpr bool NEEDROWS = (Tr::MULT && (MUL == CM_PROP || MUL == CM_PROPCONJ));
+  constexpr bool NEEDROWS =
+      (Tr::MULT && (MUL == CM_PROP || MUL == CM_PROPCONJ || MUL == CM_PROP_ZB || MUL == CM_PROPCONJ_ZB));
+  constexpr bool CONJ = MUL == CM_PROPCONJ || MUL == CM_PROPCONJ_ZB;
+  // the propagator zeroes rows N/4 .. 3N/4 - 1 (canonical blocks E/4 .. 3E/4 - 1)
+  constexpr bool ZB = (MUL == CM_PROP_ZB || MUL == CM_PROPCONJ_ZB) && E % 4 == 0 && P::R1 % 4 == 0;
   float kxv = 0.f;
   if constexpr (NEEDK) kxv = a.kx[x];
   int2 pr = make_int2(0, 0);
@@ -488,7 +414,7 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, co
     for (int c = 0; c < E; ++c) {
       const int r = t + c * T;
       float2 p = cmul(pxv, a.py[r]);
-      if constexpr (MUL == CM_PROPCONJ) p.y = -p.y;
+      if constexpr (CONJ) p.y = -p.y;
       const float m = (r <= pr.x || r >= pr.y) ? a.scale : 0.f;
       v[c] = cmul(v[c], cscale(p, m));
     }
@@ -504,7 +430,7 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, co
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = cscale(v[c], a.scale);
   }
-  if constexpr (Tr::INV) fft_line<P, N, +1, S>(v, t, tw, L);
+  if constexpr (Tr::INV) fft_line<P, N, +1, S, LL, ZB>(v, t, tw, L);
 }
 
 // variants 0, 1: the C columns of a tile spread over the workgroup, thread
@@ -655,6 +581,8 @@ static hipError_t col_wl(int mode, int mul, const ColArgs& a, hipStream_t s) {
   else if (mode == COL_INV) MSL_COLW(COL_INV, CM_SCALE);
   else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLW(COL_FWD_MUL_INV, CM_PROP);
   else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLW(COL_FWD_MUL_INV, CM_PROPCONJ);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP_ZB) MSL_COLW(COL_FWD_MUL_INV, CM_PROP_ZB);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ_ZB) MSL_COLW(COL_FWD_MUL_INV, CM_PROPCONJ_ZB);
   else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLW(COL_FWD_MUL_INV, CM_BAND);
   else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLW(COL_FWD_MUL_INV, CM_GRAD);
   else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COLW(COL_FWD_MUL_INV, CM_BANDGRAD);
@@ -1032,6 +960,8 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
     else if (mode == COL_INV) MSL_COL(COL_INV, CM_SCALE);
     else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COL(COL_FWD_MUL_INV, CM_PROP);
     else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COL(COL_FWD_MUL_INV, CM_PROPCONJ);
+    else if (mode == COL_FWD_MUL_INV && mul == CM_PROP_ZB) MSL_COL(COL_FWD_MUL_INV, CM_PROP_ZB);
+    else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ_ZB) MSL_COL(COL_FWD_MUL_INV, CM_PROPCONJ_ZB);
     else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COL(COL_FWD_MUL_INV, CM_BAND);
     else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COL(COL_FWD_MUL_INV, CM_GRAD);
     else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COL(COL_FWD_MUL_INV, CM_BANDGRAD);
@@ -1086,6 +1016,8 @@ static hipError_t col_stream(int mode, int mul, const ColArgs& a, hipStream_t s)
   else if (mode == COL_INV) MSL_COLS(COL_INV, CM_SCALE);
   else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLS(COL_FWD_MUL_INV, CM_PROP);
   else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP_ZB) MSL_COLS(COL_FWD_MUL_INV, CM_PROP_ZB);
+  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ_ZB) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ_ZB);
   else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLS(COL_FWD_MUL_INV, CM_BAND);
   else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLS(COL_FWD_MUL_INV, CM_GRAD);
   else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COLS(COL_FWD_MUL_INV, CM_BANDGRAD);
@@ -1134,11 +1066,23 @@ hipError_t launch_row_v(int mode, int mul, const RowArgs& a, hipStream_t s, int 
 }
 
 static bool needs_prop_rows(int mode, int mul) {
-  return (mul == CM_PROP || mul == CM_PROPCONJ) && mode != COL_FWD && mode != COL_INV;
+  return (mul == CM_PROP || mul == CM_PROPCONJ || mul == CM_PROP_ZB || mul == CM_PROPCONJ_ZB) && mode != COL_FWD &&
+         mode != COL_INV;
+}
+
+bool prop_rows_quarter(const float* hky, int ny, float kmax) {
+  if (ny < 4) return false;
+  const float kyv = hky[ny / 4], kxv = 0.0f;
+  return !(std::sqrt(kxv * kxv + kyv * kyv) < kmax);
 }
 
 hipError_t launch_col_v(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
   if (needs_prop_rows(mode, mul) && !a.prop_rows) return hipErrorInvalidValue;
+  // the inverse transform after the propagator skips rows known to be zero
+  if (mode == COL_FWD_MUL_INV && a.zb_rows) {
+    if (mul == CM_PROP) mul = CM_PROP_ZB;
+    else if (mul == CM_PROPCONJ) mul = CM_PROPCONJ_ZB;
+  }
   switch (a.ny) {
     case 16: return col_n<16>(mode, mul, a, s, var);
     case 32: return col_n<32>(mode, mul, a, s, var);

@@ -279,7 +279,9 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
   {
 #pragma clang fp contract(off)
     h = a.MaxZ - a.MinZ;
-    zf = (a.mode >= 2) ? a.zplane : h - a.dz * (float)(a.first_slice + slice);
+    // compat=physical: slice i starts at the top of the specimen (i - 1 slices down)
+    const int phys = (a.mode < 2 && a.physical) ? 1 : 0;
+    zf = (a.mode >= 2) ? a.zplane : h - a.dz * (float)(a.first_slice + slice - phys);
   }
   int k0, k1;
   if (a.mode == 1) {
@@ -289,6 +291,12 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
     const int kc = (int)floorf((h - zf) / 1.0f);
     k0 = kc - a.loadslicesz;
     k1 = kc + a.loadslicesz;
+    if (a.mode == 0 && a.physical) {
+      // down to the slab of the lowest sample, +-3 (every atom within sqrt(5))
+#pragma clang fp contract(off)
+      const float zb = zf - a.dz;
+      k1 = (int)floorf((h - zb) / 1.0f) + a.loadslicesz;
+    }
   }
   k0 = max(k0, 0);
   k1 = min(k1, a.nslab - 1);
@@ -458,7 +466,7 @@ __device__ float block_sum(float v, float* sh) {
 // per wave: partial[b][blk] = sum |w| over a fixed 4096-element chunk
 constexpr int L1_CHUNK = 4096;
 __global__ void __launch_bounds__(RB) k_abs_partial(const float2* w, long long n, long long bstride, float* partial,
-                                                    int nchunk) {
+                                                    int nchunk, int l2) {
   __shared__ float sh[RB];
   const int b = blockIdx.y, blk = blockIdx.x;
   const float2* p = w + (long long)b * bstride;
@@ -466,19 +474,22 @@ __global__ void __launch_bounds__(RB) k_abs_partial(const float2* w, long long n
   const long long s0 = (long long)blk * L1_CHUNK;
   for (int i = threadIdx.x; i < L1_CHUNK; i += RB) {
     const long long j = s0 + i;
-    if (j < n) acc += sqrtf(p[j].x * p[j].x + p[j].y * p[j].y);
+    if (j < n) {
+      const float i2 = p[j].x * p[j].x + p[j].y * p[j].y;
+      acc += l2 ? i2 : sqrtf(i2);
+    }
   }
   const float r = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[(long long)b * nchunk + blk] = r;
 }
 
-__global__ void __launch_bounds__(RB) k_abs_final(float* partial, int nchunk, float* norm) {
+__global__ void __launch_bounds__(RB) k_abs_final(float* partial, int nchunk, float* norm, int l2) {
   __shared__ float sh[RB];
   const int b = blockIdx.x;
   float acc = 0.f;
   for (int i = threadIdx.x; i < nchunk; i += RB) acc += partial[(long long)b * nchunk + i];
   const float r = block_sum(acc, sh);
-  if (threadIdx.x == 0) norm[b] = r;
+  if (threadIdx.x == 0) norm[b] = l2 ? sqrtf(r) : r;
 }
 
 __global__ void k_div_by(float2* w, long long n, long long bstride, const float* norm) {
@@ -491,11 +502,16 @@ __global__ void k_div_by(float2* w, long long n, long long bstride, const float*
 }
 
 hipError_t launch_l1_normalise(float2* w, long long n, int nbatch, long long bstride, float* scratch, hipStream_t s) {
+  return launch_probe_normalise(w, n, nbatch, bstride, scratch, 0, s);
+}
+
+hipError_t launch_probe_normalise(float2* w, long long n, int nbatch, long long bstride, float* scratch, int l2,
+                                  hipStream_t s) {
   const int nchunk = (int)nblk(n, L1_CHUNK);
   float* partial = scratch;
   float* norm = scratch + (long long)nbatch * nchunk;
-  hipLaunchKernelGGL(k_abs_partial, dim3(nchunk, nbatch), dim3(RB), 0, s, w, n, bstride, partial, nchunk);
-  hipLaunchKernelGGL(k_abs_final, dim3(nbatch), dim3(RB), 0, s, partial, nchunk, norm);
+  hipLaunchKernelGGL(k_abs_partial, dim3(nchunk, nbatch), dim3(RB), 0, s, w, n, bstride, partial, nchunk, l2);
+  hipLaunchKernelGGL(k_abs_final, dim3(nbatch), dim3(RB), 0, s, partial, nchunk, norm, l2);
   hipLaunchKernelGGL(k_div_by, dim3(nblk(n, 256), nbatch), dim3(256), 0, s, w, n, bstride, norm);
   return hipGetLastError();
 }
@@ -733,16 +749,21 @@ hipError_t launch_ctf(const CtfArgs& a, hipStream_t s) {
 
 // ============================================================== detectors
 // CentreKernel + conjugationKernel + host sum of AddTDSWaves (U.cu:2734-2755)
-__global__ void k_tds_accum(const float2* F, float* tds, int nx, int ny) {
+__global__ void k_tds_accum(const float2* F, float* tds, int nx, int ny, int intensity, float scale) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= nx) return;
   const int x2 = x >= nx / 2 ? x - nx / 2 : x + nx / 2;
   const int y2 = y >= ny / 2 ? y - ny / 2 : y + ny / 2;
   const float2 v = F[(long long)y2 * nx + x2];
-  tds[(long long)y * nx + x] += sqrtf(v.x * v.x + v.y * v.y);
+  const float i2 = v.x * v.x + v.y * v.y;
+  tds[(long long)y * nx + x] += intensity ? scale * i2 : sqrtf(i2);
 }
 hipError_t launch_tds_accum(const float2* F, float* tds, int nx, int ny, hipStream_t s) {
-  hipLaunchKernelGGL(k_tds_accum, dim3(nblk(nx, 256), ny), dim3(256), 0, s, F, tds, nx, ny);
+  return launch_tds_accum_mode(F, tds, nx, ny, 0, 1.0f, s);
+}
+hipError_t launch_tds_accum_mode(const float2* F, float* tds, int nx, int ny, int intensity, float scale,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(k_tds_accum, dim3(nblk(nx, 256), ny), dim3(256), 0, s, F, tds, nx, ny, intensity, scale);
   return hipGetLastError();
 }
 

@@ -145,6 +145,7 @@ ColArgs col_args(msl_sim* s, const float2* src, float2* dst, int nbatch, long lo
   a.kmax = s->kmax;
   a.scale = 1.0f;
   a.tw = s->twy.p;
+  a.zb_rows = s->zb_rows ? 1 : 0;
   return a;
 }
 
@@ -324,12 +325,15 @@ void precalc(msl_sim* s) {
   const float hx = s->hkx[imidx - 1] * 1 / 2, hy = s->hky[imidy - 1] * 1 / 2;
   const float kmx = (float)std::pow(hx, 2), kmy = (float)std::pow(hy, 2);
   s->kmax = std::sqrt((kmy <= kmx) ? kmy : kmx);
+  // compat=physical: 2/3 of the Nyquist frequency
+  if (s->compat & MSL_COMPAT_PHYSICAL) s->kmax = (2.0f / 3.0f) * std::min(s->hkx[imidx], s->hky[imidy]);
   s->nf = 1.0f / std::sqrt((float)(rx * ry));
   // band pruning is exact when every column N/4 <= x < 3N/4 fails the
   // propagator's |k| < kmax test for any ky (then so does the band limit's)
   s->prune_tiles = band_tiles_lo(rx);
   for (int x = rx / 4; x < 3 * rx / 4 && s->prune_tiles > 0; ++x)
     if (std::sqrt(s->hkx[x] * s->hkx[x] + 0.0f) < s->kmax) s->prune_tiles = 0;
+  s->zb_rows = prop_rows_quarter(s->hky.data(), ry, s->kmax);
   s->kx.ensure(rx);
   s->ky.ensure(ry);
   HIPC(hipMemcpyAsync(s->kx.p, s->hkx.data(), rx * sizeof(float), hipMemcpyHostToDevice, s->stream));
@@ -370,6 +374,7 @@ PotArgs pot_args(msl_sim* s, int mode, int first, int nslices, float dz, float z
   p.nx = s->resX; p.ny = s->resY; p.ps = s->ps; p.offx = s->offx; p.offy = s->offy;
   p.dz = dz; p.first_slice = first; p.nslices = nslices; p.zplane = zplane;
   p.loadslicesz = s->loadslicesz; p.sigma = s->sigma2; p.mode = mode; p.out_kind = out_kind;
+  p.physical = (s->compat & MSL_COMPAT_PHYSICAL) ? 1 : 0;
   p.out = out; p.out_sstride = sstride;
   return p;
 }
@@ -398,7 +403,8 @@ void make_probe(msl_sim* s, float2* out, int posx, int posy) {
   p.ps = s->ps; p.df = s->df; p.Cs = s->Cs; p.wavel = s->wavel;
   HIPC(launch_probe_kspace(p, s->stream));
   fft2(s, out, 1, true);
-  HIPC(launch_l1_normalise(out, (long long)s->npix(), 1, (long long)s->npix(), s->red.p, s->stream));
+  HIPC(launch_probe_normalise(out, (long long)s->npix(), 1, (long long)s->npix(), s->red.p,
+                              (s->compat & MSL_COMPAT_PHYSICAL) ? 1 : 0, s->stream));
 }
 
 float download_scalar(msl_sim* s, const float* d) {
@@ -892,7 +898,8 @@ msl_status msl_add_tds_waves(msl_sim* s) {
   require(s->have_tds && s->have_waves, MSL_ERR_STATE, "AllocTDS and a wave first");
   HIPC(hipMemcpyAsync(s->B.p, s->Psi.p, s->npix() * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
   fft2(s, s->B.p, 1, false);
-  HIPC(launch_tds_accum(s->B.p, s->tds.p, s->resX, s->resY, s->stream));
+  const bool phys = (s->compat & MSL_COMPAT_PHYSICAL) != 0;
+  HIPC(launch_tds_accum_mode(s->B.p, s->tds.p, s->resX, s->resY, phys ? 1 : 0, s->nf * s->nf, s->stream));
   API_END(s)
 }
 
@@ -1302,8 +1309,9 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
     pacbed.ensure(np);
     HIPC(hipMemsetAsync(pacbed.p, 0, np * sizeof(float), s->stream));
   }
-  // probe normalisation: sum |psi| of the probe at the origin (U.cu:2698-2705);
-  // integer-pixel shifts leave it unchanged
+  // probe normalisation: sum |psi| of the probe at the origin (U.cu:2698-2705;
+  // sqrt(sum |psi|^2) under compat=physical); integer-pixel shifts leave it
+  // unchanged
   float2* tmp = s->B.p;
   {
     ProbeArgs p{};
@@ -1313,7 +1321,7 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
     HIPC(launch_probe_kspace(p, s->stream));
     fft2(s, tmp, 1, true);
   }
-  HIPC(launch_l1_normalise(tmp, np, 1, np, s->red.p, s->stream));
+  HIPC(launch_probe_normalise(tmp, np, 1, np, s->red.p, (s->compat & MSL_COMPAT_PHYSICAL) ? 1 : 0, s->stream));
   const int l1chunks = (int)((np + 4095) / 4096);
   const float norm = download_scalar(s, s->red.p + l1chunks);
   require(norm > 0 && std::isfinite(norm), MSL_ERR_RANGE, "probe has zero amplitude (aperture too small?)");

@@ -97,6 +97,7 @@ struct msl_sim {
   bool have_calc = false, have_freqs = false;
   float kmax = 0, nf = 0;
   int prune_tiles = 0;  // > 0: columns N/4 <= x < 3N/4 are outside the propagator band
+  bool zb_rows = false;  // rows N/4 <= y < 3N/4 are outside the propagator band (every column)
   std::vector<float> hkx, hky;
   mslh::DBuf<float> kx, ky;
   mslh::DBuf<float2> twx, twy, px, py;

@@ -53,7 +53,11 @@ struct RowArgs {
 enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FWD_MUL_INV = 2, COL_FWD_MUL = 3, COL_FWD_DETECT = 4, COL_MUL_INV = 5 };
 // k-space factor: none (scale only), Fresnel propagator with strict
 // |k| < kmax, band limit |k| <= kmax, spectral Laplacian, conjugate propagator
-enum ColMul { CM_SCALE = 0, CM_PROP = 1, CM_BAND = 2, CM_GRAD = 3, CM_PROPCONJ = 4, CM_BANDGRAD = 5 };
+enum ColMul { CM_SCALE = 0, CM_PROP = 1, CM_BAND = 2, CM_GRAD = 3, CM_PROPCONJ = 4, CM_BANDGRAD = 5,
+              // CM_PROP / CM_PROPCONJ where rows ny/4 .. 3ny/4 - 1 lie outside the band for every
+              // column (ColArgs::zb_rows): the inverse transform skips their zeros.  Chosen by
+              // launch_col from zb_rows; callers pass CM_PROP / CM_PROPCONJ.
+              CM_PROP_ZB = 6, CM_PROPCONJ_ZB = 7 };
 // CM_BANDGRAD: band limit and spectral Laplacian in one factor (the Laplacian
 // of a band-limited wave, from the spectrum before the band limit)
 
@@ -79,7 +83,9 @@ enum ColMul { CM_SCALE = 0, CM_PROP = 1, CM_BAND = 2, CM_GRAD = 3, CM_PROPCONJ =
   float G* inten;                                                                                        \
   /* > 0: launch only the tiles_lo column tiles at each end of kx (band-limited data); */                \
   /* COL_FWD_DETECT partials are then indexed by launched tile */                                        \
-  int tiles_lo;
+  int tiles_lo;                                                                                          \
+  /* 1: rows ny/4 .. 3ny/4 - 1 are outside |k| < kmax for every column (prop_rows_quarter) */            \
+  int zb_rows;
 struct ColArgs {
   MSL_COL_ARGS()
 };
@@ -89,6 +95,10 @@ struct ColArgs {
 // band is not a run from each end of ky
 hipError_t make_prop_rows(const float* kx, const float* ky, int nx, int ny, float kmax, int2* rows, hipStream_t s);
 int col_tile(int ny);  // columns per column-pass workgroup (detector layout)
+// true when rows ny/4 .. 3ny/4 - 1 fail the propagator test |k| < kmax for
+// every column: the row ny/4 fails it at kx = 0 (the FP32 test of
+// k_prop_rows; |ky| only grows towards ny/2 and |k| >= |ky|)
+bool prop_rows_quarter(const float* hky, int ny, float kmax);
 int band_tiles_lo(int nx);  // column tiles per end covering x < N/4, x >= 3N/4 (0: no pruning)
 // pass layout variants per length (kind 0 row, 1 column), see k_pass.hip
 int pass_variant(int kind, int n);
@@ -165,6 +175,7 @@ hipError_t launch_bin(const BinArgs& a, hipStream_t s);
   int first_slice, nslices;  /* slice i uses z = (MaxZ-MinZ) - dz * i (modes 0, 1) */                    \
   float zplane;              /* modes 2, 3: plane position (single slice) */                             \
   int loadslicesz;                                                                                       \
+  int physical;              /* modes 0, 1: compat=physical slice position and slab window */            \
   float sigma;                                                                                           \
   /* mode 0 3D-integrated slice, 1 projected, 2 FD plane, 3 real-space 1 A slab; */                     \
   /* out_kind 0 exp(i sigma V), 1 (V, 0), 2 V */                                                         \
@@ -200,6 +211,10 @@ hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s);
 
 // sum |w| over n elements per wave (deterministic), then w /= sum.
 hipError_t launch_l1_normalise(float2* w, long long n, int nbatch, long long bstride, float* scratch, hipStream_t s);
+// as launch_l1_normalise, dividing by sqrt(sum |w|^2) when l2 != 0; the
+// divisor of wave b lands in scratch[nbatch * ceil(n / 4096) + b]
+hipError_t launch_probe_normalise(float2* w, long long n, int nbatch, long long bstride, float* scratch, int l2,
+                                  hipStream_t s);
 hipError_t launch_fill(float2* w, long long n, float2 v, hipStream_t s);
 hipError_t launch_scale(float2* w, long long n, float s, hipStream_t st);
 hipError_t launch_cmul(const float2* a, const float2* b, float2* out, long long n, float s, hipStream_t st);
@@ -234,6 +249,9 @@ hipError_t launch_intensity(const float2* w, float* out, long long n, float s, h
 
 // TDS / CBED: tds[y][x] += |F[(y+ny/2)%ny][(x+nx/2)%nx]| (U.cu:2734-2766)
 hipError_t launch_tds_accum(const float2* F, float* tds, int nx, int ny, hipStream_t s);
+// intensity != 0: tds += scale |F|^2 (compat=physical) instead of |F|
+hipError_t launch_tds_accum_mode(const float2* F, float* tds, int nx, int ny, int intensity, float scale,
+                                 hipStream_t s);
 // sum over nbatch maps of n floats in fixed order: out (+)= sum_b in[b]
 hipError_t launch_sum_maps(const float* in, int nbatch, long long n, float* out, int accumulate, hipStream_t s);
 

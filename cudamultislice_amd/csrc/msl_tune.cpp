@@ -40,6 +40,7 @@ struct Bench {
     const float ks = 0.1f * n;
     for (int i = 0; i < n; ++i) kk[i] = (i <= n / 2 ? (float)i : (float)(i - n)) / ks;
     kmax = 0.5f * (n / 2 - 1) / ks;
+    kx = kk;
     for (int i = 0; i < n; ++i) {
       const double a = 3.14159265 * 2.0 * 0.0197 * (double)kk[i] * kk[i];
       p[i] = make_float2((float)std::cos(a), (float)-std::sin(a));
@@ -72,6 +73,7 @@ struct Bench {
     c.kx = k.as<float>(); c.ky = k.as<float>(); c.px = pxy.as<float2>(); c.py = pxy.as<float2>();
     c.kmax = kmax; c.scale = 1.0f / ((float)n * n); c.tw = tw.as<float2>(); c.prop_rows = rows.as<int2>();
     c.tiles_lo = tiles_lo;
+    c.zb_rows = prop_rows_quarter(kx.data(), n, kmax) ? 1 : 0;
     return launch_col_v(COL_FWD_MUL_INV, CM_PROP, c, s, var);
   }
 };

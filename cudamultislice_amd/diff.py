@@ -44,6 +44,10 @@ class DiffEngine:
         lib().msl_diff_info(self._h, C.byref(wl), C.byref(sg), C.byref(km), C.byref(ws))
         return {"wavelength": wl.value, "sigma": sg.value, "kmax": km.value, "workspace_bytes": ws.value}
 
+    def set_chunk(self, probes: int) -> None:
+        """probes per adjoint chunk of the backward (0: automatic)"""
+        self._check(lib().msl_diff_set_chunk(self._h, int(probes)))
+
     def _check(self, st):
         if st != 0:
             raise MslError(st, lib().msl_diff_last_error(self._h).decode())

@@ -18,14 +18,20 @@ def _p(a):
 
 
 class MultisliceSimulation:
-    def __init__(self, device: int = 0, compat_symbols: bool = True):
+    COMPAT_REFERENCE_SYMBOLS = 1
+    COMPAT_PHYSICAL = 2
+
+    def __init__(self, device: int = 0, compat_symbols: bool = True, physical: bool = False):
+        """physical: compat=physical, the reference quirks of slice position,
+        slab window, band limit, probe norm and CBED fixed (msl.h)"""
         h = C.c_void_p()
         st = lib().msl_create(C.byref(h), device)
         if st != 0:
             raise MslError(st, "msl_create failed (no HIP device?)")
         self._h = h
-        if not compat_symbols:
-            self._c(lib().msl_set_compat(self._h, 0))
+        flags = (self.COMPAT_REFERENCE_SYMBOLS if compat_symbols else 0) | (self.COMPAT_PHYSICAL if physical else 0)
+        if flags != self.COMPAT_REFERENCE_SYMBOLS:
+            self._c(lib().msl_set_compat(self._h, flags))
 
     def close(self):
         if getattr(self, "_h", None):

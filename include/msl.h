@@ -41,7 +41,24 @@ enum {
 
 /* compatibility flags (msl_set_compat) */
 enum {
-  MSL_COMPAT_REFERENCE_SYMBOLS = 1 /* XYZ symbols outside GetZNum's table map to H (U.cu:1188-1243) */
+  MSL_COMPAT_REFERENCE_SYMBOLS = 1, /* XYZ symbols outside GetZNum's table map to H (U.cu:1188-1243) */
+  /* compat=physical: reference quirks fixed instead of reproduced (default off):
+   *  - conventional slice i integrates [H - i dz, H - (i-1) dz), the top slab
+   *    included (the reference starts one slice lower, U.cu:2097, 636);
+   *  - the integrated potential reads every slab within sqrt(5) A of the
+   *    slice's samples (the reference's +-3 one-angstrom slabs, U.cu:602-605,
+   *    truncate slices thicker than ~0.76 A);
+   *  - band limit (propagator, FD band limit) at 2/3 of the Nyquist frequency
+   *    (U.cu:1929-1953 takes about half of it);
+   *  - STEM probes normalised to sum |psi|^2 = 1 (U.cu:2698-2705 divides by
+   *    sum |psi|);
+   *  - AddTDSWaves accumulates |FFT psi|^2 / (nx ny) (U.cu:2741-2755
+   *    accumulates |FFT psi|).
+   * The band limit follows the flag at the next PreCalculateFrequencies, the
+   * rest at the next potential, probe or TDS call.  Single-handle and batched
+   * STEM paths; the serving and differentiable engines keep the reference's
+   * behaviour. */
+  MSL_COMPAT_PHYSICAL = 2
 };
 
 /* potential modes (msl_set_potential_mode) */
@@ -241,6 +258,10 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
 msl_status msl_diff_forward_only(msl_diff* d, const float* V, const void* psi0, float* intensity, void* stream);
 msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream);
 msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, float* kmax, long long* workspace_bytes);
+/* probes per adjoint chunk of the backward (0: automatic, the probes whose
+ * band-limited gradient waves fill 128 MiB); every chunk runs through a whole
+ * segment before the next, and gradient rows are summed in chunk order */
+msl_status msl_diff_set_chunk(msl_diff* d, int probes);
 
 /* ---- pass layouts and autotuning (no counterpart in the reference) ----
  * kind 0: row pass, 1: column pass.  Variants are alternative LDS layouts and

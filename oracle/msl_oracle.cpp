@@ -47,7 +47,7 @@ int stem_batch(Sim<Real>& s, int nprobe, const int* posx, const int* posy, float
   s.conventionaldz = dz;
   std::vector<std::vector<C>> t(nslices);
   for (int i = 1; i <= nslices; ++i) {
-    Real z = Real(s.slice_z(dz, i));
+    Real z = Real(s.conv_z(dz, i));
     if (potmode == 1) s.potential_proj(z, Real(dz), t[i - 1]);
     else s.potential_conv(z, Real(dz), t[i - 1]);
   }
@@ -99,6 +99,38 @@ int stem_batch(Sim<Real>& s, int nprobe, const int* posx, const int* posy, float
       cbed[k] = acc;
     }
   }
+  return 0;
+}
+
+// Batched finite-difference STEM over the binned atoms held: every probe
+// starts as STEMInitialiseWavefunctions does (probe, then one propagator
+// step of dzFD for the second wave, U.cu:2672-2724) and runs nsteps
+// MultisliceStep (U.cu:2040-2083); the band-limited potential of each step
+// is computed once and shared.  Threads split probes.  adf[p] as in
+// stem_batch.
+template <class Real>
+int fd_batch(Sim<Real>& s, int nprobe, const int* posx, const int* posy, int nsteps, float adf_in, float adf_out,
+             double* adf, int nthreads) {
+  using C = std::complex<Real>;
+  std::vector<std::vector<C>> V(nsteps);
+  for (int it = 1; it <= nsteps; ++it) s.fd_potential(it, V[it - 1]);
+  int saved = num_threads();
+  int nt = nthreads > 0 ? nthreads : saved;
+  num_threads() = 1;  // inner FFTs single-threaded; parallel over probes
+  auto worker = [&](int tid) {
+    std::vector<C> Pm, Ps, Pp;
+    for (int p = tid; p < nprobe; p += nt) {
+      s.probe(posx[p], posy[p], Pm);
+      Ps = Pm;
+      s.propagate(Ps, Real(s.dzFD));
+      for (int it = 0; it < nsteps; ++it) s.fd_update(V[it], Pm, Ps, Pp);
+      adf[p] = (double)s.adf(Ps, adf_in, adf_out);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int i = 0; i < nt; ++i) th.emplace_back(worker, i);
+  for (auto& x : th) x.join();
+  num_threads() = saved;
   return 0;
 }
 
@@ -314,6 +346,10 @@ void orc_init_stem(void* p, int px, int py, float dz) {
 }
 void orc_step_conv(void* p, int it, int potmode) { dispatch((Handle*)p, [&](auto& s) { s.step_conv(it, potmode); return 0; }); }
 void orc_step_fd(void* p, int it) { dispatch((Handle*)p, [&](auto& s) { s.step_fd(it); return 0; }); }
+// compat=physical (1) or the reference's behaviour (0); before set_calc
+void orc_set_physical(void* p, int on) {
+  dispatch((Handle*)p, [&](auto& s) { s.physical = on != 0; return 0; });
+}
 // a fixed real potential for step_fd (NULL restores the atoms' potential)
 void orc_set_fd_potential(void* p, const double* V) {
   dispatch((Handle*)p, [&](auto& s) {
@@ -375,6 +411,12 @@ void orc_ctf_image(void* p, double df, double* out) {
   });
 }
 
+int orc_fd_batch(void* p, int nprobe, const int* posx, const int* posy, int nsteps, float adf_in, float adf_out,
+                 double* adf, int nthreads) {
+  return dispatch((Handle*)p, [&](auto& s) {
+    return fd_batch(s, nprobe, posx, posy, nsteps, adf_in, adf_out, adf, nthreads);
+  });
+}
 int orc_stem_batch(void* p, int nprobe, const int* posx, const int* posy, float dz, int nslices, int potmode,
                    float adf_in, float adf_out, double* adf, double* exitw, double* cbed, int nthreads) {
   return dispatch((Handle*)p, [&](auto& s) {

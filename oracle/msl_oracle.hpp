@@ -356,6 +356,19 @@ template <class Real> struct Sim {
   std::vector<C> PsiMinus, Psi, PsiPlus, prop;
   std::vector<Real> TDS;
   std::vector<Real> image;
+  // compat=physical: the reference quirks below fixed instead of reproduced
+  //   Q2 slice i integrates [H - i dz, H - (i-1) dz) (the reference starts one
+  //      slice lower, U.cu:2097, and never integrates the top slab)
+  //   Q3 the slab window reaches every atom within sqrt(5) A of the slice's
+  //      samples (the reference's fixed +-3 one-angstrom slabs, U.cu:602-605,
+  //      truncate slices thicker than ~0.76 A)
+  //   Q5 band limit at 2/3 of the Nyquist frequency (U.cu:1929-1953 uses about
+  //      half of it)
+  //   Q6 probe normalised to unit intensity sum |psi|^2 = 1 (U.cu:2698-2705
+  //      divides by sum |psi|)
+  //   Q7 CBED accumulates the unitary diffracted intensity |FFT psi|^2 / (nx ny)
+  //      (U.cu:2741-2755 accumulates the unnormalised amplitude)
+  bool physical = false;
 
   size_t npix() const { return (size_t)resX * resY; }
 
@@ -484,6 +497,7 @@ template <class Real> struct Sim {
     float kmx = (float)std::pow(hx, 2), kmy = (float)std::pow(hy, 2);
     kmax = (kmy <= kmx) ? kmy : kmx;
     kmax = std::sqrt(kmax);
+    if (physical) kmax = (2.0f / 3.0f) * std::min(kx[imidx], ky[imidy]);
     fft.init(resX, resY);
     nf = Real(1) / std::sqrt(Real(resX) * Real(resY));
   }
@@ -545,6 +559,9 @@ template <class Real> struct Sim {
     float d = dz * (float)it;
     return h - d;
   }
+  // top of conventional slice `it`: slice_z, or one slice higher under
+  // compat=physical (Q2)
+  float conv_z(float dz, int it) const { return slice_z(dz, physical ? it - 1 : it); }
   // slab index floor((MaxZ - MinZ - z) / 1), decided in FP32 as in U.cu:602
   int slab_of(float z) const {
     float h = MaxZ - MinZ;
@@ -555,7 +572,9 @@ template <class Real> struct Sim {
   void potential_conv(Real z, Real convdz, std::vector<C>& out, int mode = 0) const {
     out.assign(npix(), C(0, 0));
     int kc = slab_of((float)z);
-    int k0 = std::max(0, kc - loadslicesz), k1 = std::min(nSlab - 1, kc + loadslicesz);
+    // Q3 (compat=physical): down to the slab of the lowest sample, +-3
+    int kb = physical ? slab_of((float)z - (float)convdz) : kc;
+    int k0 = std::max(0, kc - loadslicesz), k1 = std::min(nSlab - 1, kb + loadslicesz);
     parallel_for(resY, [&](int yb, int ye) {
       for (int y = yb; y < ye; ++y)
         for (int x = 0; x < resX; ++x) {
@@ -760,7 +779,12 @@ template <class Real> struct Sim {
       }
     fft.run(w.data(), true);
     Real s = 0;
-    for (auto& v : w) s += std::abs(v);
+    if (physical) {  // Q6: unit intensity
+      for (auto& v : w) s += std::norm(v);
+      s = std::sqrt(s);
+    } else {
+      for (auto& v : w) s += std::abs(v);
+    }
     for (auto& v : w) v /= s;
   }
 
@@ -774,7 +798,7 @@ template <class Real> struct Sim {
 
   // MultisliceStepConv, U.cu:2085-2124.
   void step_conv(int it, int potmode = 0) {
-    Real z = Real(slice_z(conventionaldz, it));
+    Real z = Real(conv_z(conventionaldz, it));
     std::vector<C> t;
     if (potmode == 1) potential_proj(z, Real(conventionaldz), t);
     else potential_conv(z, Real(conventionaldz), t);
@@ -803,8 +827,16 @@ template <class Real> struct Sim {
   std::vector<Real> fd_potential_override;
 
   void step_fd(int it) {
-    Real z = Real(slice_z(dzFD, it));
     std::vector<C> V;
+    fd_potential(it, V);
+    fd_update(V, PsiMinus, Psi, PsiPlus);
+  }
+
+  // The band-limited real potential of finite-difference step `it`
+  // (U.cu:2050-2063): the atoms' point-sampled potential at z = H - dzFD it
+  // (or the test hook's fixed one), FFT, band limit, IFFT / (nx ny).
+  void fd_potential(int it, std::vector<C>& V) const {
+    Real z = Real(slice_z(dzFD, it));
     if (fd_potential_override.size() == npix()) {
       V.resize(npix());
       for (size_t i = 0; i < npix(); ++i) V[i] = C(fd_potential_override[i], 0);
@@ -815,7 +847,14 @@ template <class Real> struct Sim {
     bandlimit(V);
     fft.run(V.data(), true);
     for (auto& v : V) v *= nf * nf;
-    std::vector<C> G = Psi;
+  }
+
+  // One finite-difference update (U.cu:2064-2081) from the band-limited
+  // potential V: Laplacian G of psi (gradKernel), the update of
+  // finiteDifferenceKernel into psi_+, band limit of psi_+; then psi_- <- psi,
+  // psi <- psi_+.
+  void fd_update(const std::vector<C>& V, std::vector<C>& Pm, std::vector<C>& Ps, std::vector<C>& Pp) const {
+    std::vector<C> G = Ps;
     fft.run(G.data(), false);
     Real p9 = Real(3.14159);
     for (int y = 0; y < resY; ++y)
@@ -828,16 +867,17 @@ template <class Real> struct Sim {
     Real dz = Real(dzFD);
     C cPlus(1, Real(2) * p9 * dz / wavel), cMinus(1, -Real(2) * p9 * dz / wavel);
     C rcp = Real(1) / cPlus, ratio = cMinus / cPlus;
+    Pp.resize(npix());
     for (size_t i = 0; i < npix(); ++i) {
-      C a = Real(2) * Psi[i] - dz * dz * G[i] - dz * dz * Real(4) * p9 * sigma2 * V[i].real() * Psi[i] / wavel;
-      PsiPlus[i] = rcp * a - ratio * PsiMinus[i];
+      C a = Real(2) * Ps[i] - dz * dz * G[i] - dz * dz * Real(4) * p9 * sigma2 * V[i].real() * Ps[i] / wavel;
+      Pp[i] = rcp * a - ratio * Pm[i];
     }
-    fft.run(PsiPlus.data(), false);
-    bandlimit(PsiPlus);
-    fft.run(PsiPlus.data(), true);
-    for (auto& v : PsiPlus) v *= nf * nf;
-    PsiMinus = Psi;
-    Psi = PsiPlus;
+    fft.run(Pp.data(), false);
+    bandlimit(Pp);
+    fft.run(Pp.data(), true);
+    for (auto& v : Pp) v *= nf * nf;
+    Pm = Ps;
+    Ps = Pp;
   }
 
   // AddTDSWaves, U.cu:2734-2766: TDS += |fftshift(FFT_unnormalised(psi))|.
@@ -849,7 +889,8 @@ template <class Real> struct Sim {
       for (int x = 0; x < resX; ++x) {
         int x2 = (x >= resX / 2) ? x - resX / 2 : x + resX / 2;
         int y2 = (y >= resY / 2) ? y - resY / 2 : y + resY / 2;
-        TDS[(size_t)y * resX + x] += std::abs(F[(size_t)y2 * resX + x2]);
+        const C f = F[(size_t)y2 * resX + x2];
+        TDS[(size_t)y * resX + x] += physical ? std::norm(f) * nf * nf : std::abs(f);  // Q7
       }
   }
 

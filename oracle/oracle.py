@@ -50,6 +50,7 @@ def _lib():
         "orc_init_stem": ([vp, i, i, f], None),
         "orc_step_conv": ([vp, i, i], None), "orc_step_fd": ([vp, i], None), "orc_step_rs": ([vp, i], None),
         "orc_set_fd_potential": ([vp, vp], None),
+        "orc_set_physical": ([vp, i], None),
         "orc_rs_laplacian": ([vp, _dp, _dp], None),
         "orc_probe": ([vp, i, i, _dp], None),
         "orc_get_wave": ([vp, i, _dp], None), "orc_set_wave": ([vp, i, _dp], None),
@@ -57,6 +58,7 @@ def _lib():
         "orc_adf": ([vp, f, f], d),
         "orc_ctf_image": ([vp, d, _dp], None),
         "orc_stem_batch": ([vp, i, _ip, _ip, f, i, i, f, f, vp, vp, vp, i], i),
+        "orc_fd_batch": ([vp, i, _ip, _ip, i, f, f, _dp, i], i),
         "orc_diff": ([vp, i, _fp, i, _dp, _dp, vp, vp, f], i),
         "orc_bessk0": ([d], d), "orc_bessi0": ([d], d),
         "orc_vatom": ([i, d, _fp], d), "orc_vzatom": ([i, d, _fp], d),
@@ -154,9 +156,14 @@ def tds_displacements(seed: int, config: int, n: int, sigma: float):
 class OracleSim:
     """Mirror of UnmanagedMultisliceSimulation (U.cuh:62-199) on the CPU."""
 
-    def __init__(self, use_float: bool = False):
+    def __init__(self, use_float: bool = False, physical: bool = False):
+        """physical: compat=physical, the reference quirks Q2, Q3, Q5, Q6 and
+        Q7 fixed (top slab integrated, full slab window, 2/3-Nyquist band
+        limit, unit-intensity probe, CBED of unitary intensity)"""
         self._h = L().orc_create(1 if use_float else 0)
         self.resX = self.resY = 0
+        if physical:
+            L().orc_set_physical(self._h, 1)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -292,6 +299,14 @@ class OracleSim:
         if want_cbed:
             out["cbed"] = cb.reshape(self.resY, self.resX)
         return out
+
+    def fd_batch(self, posx, posy, nsteps, adf_in=60.0, adf_out=200.0, nthreads=0):
+        """ADF of finite-difference STEM runs (STEMInitialiseWavefunctions then
+        nsteps MultisliceStep) from every probe position, threads over probes"""
+        posx = np.ascontiguousarray(posx, np.int32); posy = np.ascontiguousarray(posy, np.int32)
+        adf = np.zeros(posx.size)
+        L().orc_fd_batch(self._h, posx.size, posx, posy, int(nsteps), adf_in, adf_out, adf, nthreads)
+        return adf
 
     def diff(self, V, psi0, dLdI=None, dz=2.0):
         V = np.ascontiguousarray(V, np.float32)

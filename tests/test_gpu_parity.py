@@ -195,41 +195,6 @@ def test_example_tem_program_under_address_sanitizer(tmp_path):
     assert r.stdout.startswith("slices ")
 
 
-def test_tem_si_1024_252_slices_full_size_properties(fp):
-    """Si 14x14x92 cells (144 256 atoms), 1024^2 over 76.02 A, 2 A slices,
-    300 kV: at full size the exit wave is finite, the band-limited
-    propagation never gains intensity (|t| = 1, |P| <= 1), and the batched
-    8-defocus CTF sweep equals single-defocus images."""
-    Z, x, y, z = S.silicon(14, 14, 92)
-    from cudamultislice_amd import MultisliceSimulation
-    g = MultisliceSimulation()
-    g.set_atoms(Z, x, y, z)
-    g.GetParameterisation()
-    g.UploadBinnedAtoms()
-    g.ApplyMicroscopeParameters(300.0, -800.0, 0.0, 0.0, 1e4, 0.0, 0.0, 20.0)
-    inf = g.info()
-    g.SetCalculationVariables(0.07424, -800.0, 1024, 1024, inf["MaximumX"] - inf["MinimumX"],
-                              inf["MaximumY"] - inf["MinimumY"], inf["MaximumZ"] - inf["MinimumZ"], 2.0, 2.0)
-    g.PreCalculateFrequencies()
-    g.InitialiseWavefunctions(2.0)
-    n = g.GetSlices(True)
-    assert n == 252
-    norms = []
-    for it in range(1, n + 1):
-        g.MultisliceStepConv(it)
-        if it in (1, 60, 130, 200, n):
-            norms.append(float(np.mean(np.abs(g.get_wave(0)) ** 2)))
-    ew = g.GetExitWave()
-    assert np.all(np.isfinite(ew))
-    assert all(b <= a * (1 + 1e-5) for a, b in zip(norms, norms[1:])), norms
-    assert 0.2 < norms[-1] <= 1.0 + 1e-5
-    dfs = np.arange(-800, 601, 200, dtype=np.float32)
-    sweep = g.ctf_sweep(dfs)
-    single = g.SimulateImage(1.0)  # df -800 from ApplyMicroscopeParameters
-    assert np.all(np.isfinite(sweep)) and rel_l2(sweep[0], single) < 1e-6
-    assert len({round(float(s.std()), 6) for s in sweep}) == len(dfs)  # defocus changes the image
-
-
 def test_tiff_exports_hold_the_handle_state(fp, tmp_path):
     """The GUI's four TIFF exports (MW.cs:684-890) read back with PIL equal
     the exit wave, the CBED accumulator and GetImValue's image."""
@@ -580,28 +545,8 @@ def _serve_mix():
 
 
 def _serve_oracle(job, fp, kV, ps, dz, ap, adf):
-    from cudamultislice_amd.serve import CONVENTIONAL
-    o = O.OracleSim()
-    o.set_atoms(job.Z, job.x, job.y, job.z)
-    o.set_fparams(fp)
-    o.bin()
-    e = o.extents()
-    stem = job.nprobe > 0
-    o.apply_microscope(kV, obj=ap if stem else 0.0)
-    o.set_calc(ps, 0.0, job.n, job.n, 1, 1, float(e[4] - e[5]), 0.0, 0.0, stem)
-    step = (lambda it: o.step_conv(it)) if job.method == CONVENTIONAL else (lambda it: o.step_fd(it))
-    if not stem:
-        o.init_plane(dz) if job.method == CONVENTIONAL else o.init_plane_fd()
-        for it in range(1, job.nslices + 1):
-            step(it)
-        return {"exit": o.wave()}
-    out = []
-    for px, py in zip(job.posx, job.posy):
-        o.init_stem(int(px), int(py), dz)
-        for it in range(1, job.nslices + 1):
-            step(it)
-        out.append(o.adf(*adf))
-    return {"adf": np.array(out)}
+    from serve_oracle import job_reference
+    return job_reference(job, fp, kV, ps, dz, ap, adf)
 
 
 def test_serving_mix_of_uneven_jobs_matches_oracle(fp):
@@ -868,3 +813,75 @@ def test_long_run_100_slices_within_1e4(fp):
         g.MultisliceStepConv(it)
         o.step_conv(it)
     assert rel_l2(g.GetExitWave(), o.wave()) < 1e-4
+
+
+# ---------------------------------------------------------------- compat=physical
+@pytest.mark.parametrize("physical", [False, True])
+def test_compat_modes_match_the_oracle_quirk_by_quirk(fp, physical):
+    """Both compat modes against the oracle in the same mode, quirk by quirk:
+    Q2/Q3 the integrated potential of 2 A slices (slice position, slab
+    window), Q5 the band limit, Q6 the probe norm, Q7 the CBED accumulator,
+    and a TEM exit wave through all slices at dz = 2 A."""
+    from cudamultislice_amd import MultisliceSimulation
+    Z, x, y, z = S.srtio3(3, 3, 4)
+    g = MultisliceSimulation(physical=physical)
+    g.set_atoms(Z, x, y, z)
+    g.GetParameterisation()
+    o = O.OracleSim(physical=physical)
+    o.set_atoms(Z, x, y, z)
+    o.set_fparams(fp)
+    g.UploadBinnedAtoms()
+    o.bin()
+    N, ps, dz = 64, 0.2, 2.0
+    _setup(g, o, 300.0, ps, N, 20.0, off=(2.0, 2.0))
+    # Q5
+    inf = g.info()
+    want_kmax = 1.0 / (3 * ps) if physical else (N // 2 - 1) / (2 * N * ps)
+    assert inf["kmax"] == pytest.approx(want_kmax, rel=1e-6)
+    assert o.scalars()["kmax"] == pytest.approx(inf["kmax"], rel=1e-6)
+    assert (inf["band_tiles"] > 0) == (not physical)
+    # Q2 / Q3: slice it of the engine is the oracle's potential at conv_z
+    e = o.extents()
+    H = np.float32(e[4] - e[5])
+    for it in (1, 3, 5):
+        zf = float(H - np.float32(np.float32(dz) * np.float32(it - (1 if physical else 0))))
+        got = g.potential_slice(it, dz, real=True)
+        want = o.potential(3, zf, dz).real
+        assert want.max() > 1.0
+        assert np.max(np.abs(got - want)) < 2e-5 * np.max(np.abs(want)), it
+    # exit wave through every slice
+    g.InitialiseWavefunctions(dz)
+    o.init_plane(dz)
+    n = g.GetSlices(True)
+    for it in range(1, n + 1):
+        g.MultisliceStepConv(it)
+        o.step_conv(it)
+    assert rel_l2(g.GetExitWave(), o.wave()) < 1e-4
+    # Q6 / Q7 on a STEM wave
+    g.ApplyMicroscopeParameters(300.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 25.0)
+    g.STEMSetCalculationVariables(ps, 0.0, N, N, 1, 1, 20.0)
+    g.PreCalculateFrequencies()
+    o.apply_microscope(300.0, obj=25.0)
+    o.set_calc(ps, 0.0, N, N, 1, 1, 20.0, 0, 0, True)
+    g.AllocTDS()
+    g.STEMInitialiseWavefunctions(20, 33)
+    g.set_conventional_dz(dz)
+    o.init_stem(20, 33, dz)
+    p = g.get_wave(1)
+    norm = np.sum(np.abs(p.astype(np.complex128)) ** 2) if physical else np.sum(np.abs(p.astype(np.complex128)))
+    assert norm == pytest.approx(1.0, rel=1e-5)
+    assert rel_l2(p, o.wave(1)) < 1e-5
+    for it in range(1, 6):
+        g.MultisliceStepConv(it)
+        o.step_conv(it)
+    g.AddTDSWaves()
+    o.add_tds()
+    assert rel_l2(g.GetTDS(), o.tds()) < 1e-4
+    # batched STEM in the same mode
+    px, py = _raster(N, 3, 17)
+    r = g.stem_batched(px, py, dz, nconfig=1, adf=(10.0, 200.0))
+    ns = int(np.ceil(H / np.float32(dz)))
+    want = o.stem_batch(px, py, dz, ns, 10.0, 200.0)["adf"]
+    assert want.min() > 0
+    assert np.max(np.abs(r["adf"] - want) / want) < 1e-4
+    g.close()

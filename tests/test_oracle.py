@@ -228,9 +228,10 @@ def _f32_fma(a, b, c):
     return np.float32(float(a) * float(b) + float(c))
 
 
-def _brute_conv(s, Z, x, y, z, zc, convdz, N, ps, off):
-    """No bins: every atom in the +-3 slab window, 20 z samples, r^2 < 5
-    (decided in FP32 as the reference kernel decides it)."""
+def _brute_conv(s, Z, x, y, z, zc, convdz, N, ps, off, window=True):
+    """No bins: every atom in the +-3 slab window (window=False: every atom),
+    20 z samples, r^2 < 5 (decided in FP32 as the reference kernel decides
+    it)."""
     e = s.extents()
     H = float(e[4] - e[5])
     kc = int(np.floor(H - zc))
@@ -247,7 +248,7 @@ def _brute_conv(s, Z, x, y, z, zc, convdz, N, ps, off):
             pxf, pyf = _f32_fma(ix, ps, off), _f32_fma(iy, ps, off)
             acc = 0.0
             for a in range(len(Z)):
-                if abs(zid[a] - kc) > 3:
+                if window and abs(zid[a] - kc) > 3:
                     continue
                 dxf, dyf = np.float32(pxf - np.float32(X[a])), np.float32(pyf - np.float32(Y[a]))
                 rxyf = _f32_fma(dyf, dyf, np.float32(dxf * dxf))
@@ -968,3 +969,143 @@ def test_aberrated_probe_phase(fp):
         assert np.all(np.abs(rel - 1.0) <= 4e-9 * np.abs(chi[inside]) + 1e-10), (Cs, df)
         assert np.max(np.abs(F[~inside])) < 1e-12 * np.abs(F[0, 0])
         assert np.sum(np.abs(p)) == pytest.approx(1.0, rel=1e-12)
+
+
+def test_fd_batch_equals_probe_by_probe_steps(fp):
+    """The probe-parallel finite-difference STEM batch gives each probe's
+    ADF of STEMInitialiseWavefunctions followed by MultisliceStep calls."""
+    Z, x, y, z = S.silicon(1, 1, 2)
+    s = _sim_with(Z, x, y, z, fp)
+    s.apply_microscope(300.0, obj=25.0)
+    s.set_calc(0.25, 0.0, 32, 32, 1, 1, float(s.extents()[4] - s.extents()[5]), 0, 0, stem=True)
+    px, py = np.array([3, 17, 30], np.int32), np.array([9, 0, 22], np.int32)
+    first = 40
+    batch = s.fd_batch(px, py, first, 5.0, 200.0)
+    for p in range(3):
+        s.init_stem(int(px[p]), int(py[p]))
+        for it in range(1, first + 1):
+            s.step_fd(it)
+        assert batch[p] == pytest.approx(s.adf(5.0, 200.0), rel=1e-12)
+    assert batch.min() > 0
+
+
+
+# ---------------------------------------------------------------- compat=physical
+def _phys(Z, x, y, z, fp):
+    s = O.OracleSim(physical=True)
+    s.set_atoms(Z, x, y, z)
+    s.bin()
+    s.set_fparams(fp)
+    s._fp = fp
+    return s
+
+
+def test_physical_window_reaches_every_atom_of_a_thick_slice(fp):
+    """Q3 (U.cu:602-605): at 2 A slices the reference's +-3 slab window
+    misses atoms within reach of the slice's lower samples; under
+    compat=physical the integrated potential equals the bin-free sum over
+    every atom (3D cutoff only), where the reference mode does not."""
+    Z = np.array([14, 38, 8, 6, 6, 14], np.int32)
+    x = np.array([5.0, 6.1, 5.4, -30.0, 40.0, 5.6], np.float32)
+    y = np.array([5.2, 4.9, 6.3, -30.0, 40.0, 5.5], np.float32)
+    # the last Si atom sits in the slab four below the slice's own, within
+    # reach of its lowest samples
+    z = np.array([3.0, 4.2, 2.1, 0.0, 12.0, 0.9], np.float32)
+    N, ps, off = 12, 0.55, 32.0
+    sp, sr = _phys(Z, x, y, z, fp), _sim_with(Z, x, y, z, fp)
+    sr._fp = fp
+    for s in (sp, sr):
+        s.apply_microscope(200.0)
+        s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+    e = sp.extents()
+    H = float(e[4] - e[5])
+    zc = float(np.float32(H - 9.0))  # samples 9-11 A below the top; the atoms sit 10-13 A down
+    want = _brute_conv(sp, Z, x, y, z, zc, 2.0, N, ps, off, window=False)
+    got = sp.potential(3, zc, 2.0).real
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(want)
+    ref = sr.potential(3, zc, 2.0).real
+    assert np.max(np.abs(ref - _brute_conv(sr, Z, x, y, z, zc, 2.0, N, ps, off))) <= 1e-12 * np.max(want)
+    assert np.max(want - ref) > 1e-4 * np.max(want)  # the reference window loses a contribution
+
+
+def test_physical_slices_integrate_the_top_slab(fp):
+    """Q2 (U.cu:2097, 636): the reference's slice i integrates
+    [H - (i+1) dz, H - i dz), so the top slab [H - dz, H] is never reached
+    (with the 2 A margin above the top atom and dz = 2 A, the upper half of
+    the top layer's potential); under compat=physical slice i covers
+    [H - i dz, H - (i-1) dz) and the slices together give the top atom's whole
+    (cut-off) projection."""
+    from scipy.integrate import quad
+    Z = np.array([38, 6, 6], np.int32)
+    x = np.array([5.0, -30.0, 40.0], np.float32)
+    y = np.array([5.0, -30.0, 40.0], np.float32)
+    z = np.array([29.6, 0.0, 28.0], np.float32)  # the Sr atom is the top atom
+    N, ps, off, dz = 16, 0.25, 33.0, 2.0
+    sums = {}
+    for phys in (False, True):
+        s = _phys(Z, x, y, z, fp) if phys else _sim_with(Z, x, y, z, fp)
+        s.apply_microscope(300.0)
+        s.set_calc(ps, 0.0, N, N, 1, 1, 1, off, off)
+        e = s.extents()
+        H = np.float32(e[4] - e[5])
+        n = int(np.ceil(H / np.float32(dz)))
+        acc = 0
+        for it in range(1, n + 1):
+            zt = float(H - np.float32(np.float32(dz) * np.float32(it - (1 if phys else 0))))
+            acc = acc + s.potential(3, zt, dz).real
+        sums[phys] = acc
+    X = float(np.float32(5.0) - e[1])
+    c = off + np.arange(N) * ps
+    ix = int(np.argmin(np.abs(c - (X + 0.7))))
+    r2 = (c[ix] - X) ** 2 + (c[ix] - X) ** 2
+    cut = quad(lambda t: O.vatom(38, r2 + t * t, fp), -np.sqrt(5 - r2), np.sqrt(5 - r2), limit=200)[0]
+    # right-end Riemann sums of step 0.1 A
+    assert sums[True][ix, ix] == pytest.approx(cut, rel=2e-2)
+    assert sums[False][ix, ix] < 0.7 * cut  # the reference loses the upper half
+
+
+def test_physical_band_limit_probe_norm_and_cbed(fp):
+    """Q5: kmax = 2/3 of the Nyquist frequency (1 / (3 ps)); Q6: the probe
+    has unit intensity; Q7: AddTDSWaves accumulates the unitary diffracted
+    intensity |FFT psi|^2 / (nx ny), centred."""
+    Z, x, y, z = S.silicon(1, 1, 1)
+    N, ps = 64, 0.25
+    sp = _phys(Z, x, y, z, fp)
+    sp.apply_microscope(300.0, obj=25.0)
+    sp.set_calc(ps, 0.0, N, N, 1, 1, 10, 0, 0, stem=True)
+    assert sp.scalars()["kmax"] == pytest.approx(1.0 / (3 * ps), rel=1e-6)
+    p = sp.probe(10, 20)
+    assert np.sum(np.abs(p) ** 2) == pytest.approx(1.0, rel=1e-12)
+    sp.set_wave(p)
+    sp.add_tds(); sp.add_tds()
+    want = 2 * np.abs(np.fft.fftshift(np.fft.fft2(p))) ** 2 / (N * N)
+    assert np.allclose(sp.tds(), want, rtol=1e-12, atol=1e-18)
+    assert sp.tds().sum() == pytest.approx(2.0, rel=1e-12)  # Parseval
+    sr = _sim_with(Z, x, y, z, fp)
+    sr.apply_microscope(300.0, obj=25.0)
+    sr.set_calc(ps, 0.0, N, N, 1, 1, 10, 0, 0, stem=True)
+    assert sr.scalars()["kmax"] == pytest.approx((N // 2 - 1) / (2 * N * ps), rel=1e-6)
+    assert np.sum(np.abs(sr.probe(10, 20))) == pytest.approx(1.0, rel=1e-12)
+
+
+def test_physical_step_uses_the_slice_below_the_top(fp):
+    """Under compat=physical MultisliceStepConv's slice 1 is the
+    transmission of [H - dz, H) followed by the propagator; in reference mode
+    the transmission of [H - 2 dz, H - dz)."""
+    Z, x, y, z = S.silicon(2, 2, 2)
+    N, ps, dz = 32, 0.4, 2.0
+    for phys in (False, True):
+        s = _phys(Z, x, y, z, fp) if phys else _sim_with(Z, x, y, z, fp)
+        s.apply_microscope(200.0)
+        s.set_calc(ps, 0.0, N, N, 1, 1, 1, 2.0, 2.0)
+        e = s.extents()
+        H = np.float32(e[4] - e[5])
+        s.init_plane(dz)
+        s.step_conv(1)
+        zt = float(H - np.float32(0.0 if phys else dz))
+        t = s.potential(0, zt, dz)
+        want = np.fft.ifft2(np.fft.fft2(t) * s.prop(dz))
+        assert np.max(np.abs(s.wave() - want)) < 1e-12
+        assert np.std(np.angle(want)) > 1e-3  # the slice holds atoms
+        waves = waves + [want] if phys else [want]
+    assert np.max(np.abs(waves[0] - waves[1])) > 1e-3
