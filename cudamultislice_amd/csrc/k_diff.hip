@@ -35,39 +35,48 @@ template <int E> __device__ __forceinline__ constexpr bool band_block(int c) {
 // REAL_Z: Z holds w_i = psi_{i-1} in real space (stored by the forward row
 // pass, or psi0), so only G is transformed; otherwise Z holds z_{i-1} in the
 // (kx, y) domain and w_i = IF_x(z_{i-1}) is recomputed here.
+// The row's transmission exp(i sigma V) is computed once per workgroup into
+// LDS, shared by its LINES lines and every probe they process (each thread
+// evaluated sincosf for its 16 elements of every line before: 3204 -> 1832
+// VALU instructions at 1024, 220 -> 190 VGPRs with w_i loaded after the first
+// transform).  Asking the compiler for three waves per SIMD spills at 1024.
 template <int N, bool REAL_Z, bool PRUNE>
 __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a) {
   using Cfg = AdjCfg<N>;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
   using LL = LdsLine<Cfg::SH>;
-  constexpr int T = Plan<N>::T, E = N / T, LINES = Cfg::L, LS = Cfg::LS;
+  constexpr int T = Plan<N>::T, E = N / T, LINES = Cfg::L, LS = Cfg::LS, NT = LINES * T;
   static_assert(LS * 2 >= N, "the reduction reuses the line buffers");
   __shared__ float2 lds[LINES * LS];
   __shared__ float2 stw[N];  // twiddles staged in LDS
+  __shared__ float2 str[N];  // transmission of row y
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   const int y = blockIdx.x, g = blockIdx.y;
-  for (int i = tid; i < N; i += LINES * T) stw[i] = a.tw[i];
+  for (int i = tid; i < N; i += NT) stw[i] = a.tw[i];
+  {
+    const float* vr = a.V + (long long)y * N;
+    // every potential value loaded before the first sincosf (whose range
+    // reduction branches would serialise the loads)
+    constexpr int K = N / NT > 0 ? N / NT : 1;
+    float v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = (tid + k * NT < N) ? vr[tid + k * NT] : 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      float sn, cs;
+      sincosf(a.sigma * v[k], &sn, &cs);
+      if (tid + k * NT < N) str[tid + k * NT] = make_float2(cs, sn);
+    }
+  }
   __syncthreads();
   const LL L{lds + ln * LS, 1};
-  const float* vr = a.V + (long long)y * N;
-  float2 tr[E];
-  // every potential value loaded before the first sincosf (whose range
-  // reduction branches would serialise the loads)
-#pragma unroll
-  for (int c = 0; c < E; ++c) tr[c].x = vr[t + c * T];
-#pragma unroll
-  for (int c = 0; c < E; ++c) {
-    float sn, cs;
-    sincosf(a.sigma * tr[c].x, &sn, &cs);
-    tr[c] = make_float2(cs, sn);
-  }
   float acc[E];
 #pragma unroll
   for (int c = 0; c < E; ++c) acc[c] = 0.f;
   for (int q = 0; q < a.per_line; ++q) {
     const int p = (g * a.per_line + q) * LINES + ln;
     const bool ok = p < a.nbatch;
-    float2 gu[E], w[E];
+    float2 gu[E];
     // lines past the batch read wave 0 and are zeroed after the loads:
     // unconditional loads issue together instead of one branch each
     const long long ro = (long long)(ok ? p : 0) * a.bstride + (long long)y * N;
@@ -76,25 +85,32 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
     for (int c = 0; c < E; ++c) {
       const bool band = !PRUNE || band_block<E>(c);
       gu[c] = band ? a.G[ro + t + c * T] : zero;
-      // the stored real-space waves are read once: past the caches, so the
-      // gradient rows stay resident for the next column pass
-      if constexpr (REAL_Z) w[c] = nt_load(&a.Z[ro + t + c * T]);
-      else w[c] = band ? a.Z[ro + t + c * T] : zero;
     }
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      gu[c] = ok ? gu[c] : zero;
-      w[c] = ok ? w[c] : zero;
-    }
+    for (int c = 0; c < E; ++c) gu[c] = ok ? gu[c] : zero;
     // band-limited inputs (PRUNE): the first butterflies skip the zero blocks
     constexpr bool ZIN = PRUNE && E % 4 == 0 && Plan<N>::R1 % 4 == 0;
-    fft_line<Plan<N>, N, +1, S, LL, ZIN>(gu, t, stw, L);              // G_u = IF_x(G_a)
-    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S, LL, ZIN>(w, t, stw, L);  // w = IF_x(z_{i-1})
+    fft_line<Plan<N>, N, +1, S, LL, ZIN>(gu, t, stw, L);  // G_u = IF_x(G_a)
+    {
+      // w_i is read only now (not held through the transform above); the
+      // stored real-space waves are read once: past the caches, so the
+      // gradient rows stay resident for the next column pass
+      float2 w[E];
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const float2 gw = cmulc(gu[c], tr[c]);     // conj(t) G_u
-      acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
-      gu[c] = gw;
+      for (int c = 0; c < E; ++c) {
+        const bool band = !PRUNE || band_block<E>(c);
+        if constexpr (REAL_Z) w[c] = nt_load(&a.Z[ro + t + c * T]);
+        else w[c] = band ? a.Z[ro + t + c * T] : zero;
+      }
+#pragma unroll
+      for (int c = 0; c < E; ++c) w[c] = ok ? w[c] : zero;
+      if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S, LL, ZIN>(w, t, stw, L);  // w = IF_x(z_{i-1})
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        const float2 gw = cmulc(gu[c], str[t + c * T]);  // conj(t) G_u
+        acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
+        gu[c] = gw;
+      }
     }
     fft_line<Plan<N>, N, -1, S>(gu, t, stw, L);              // G_{z_{i-1}} = F_x(G_w)
     if (ok) {
@@ -110,12 +126,13 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
   for (int c = 0; c < E; ++c) red[ln * N + t + c * T] = acc[c];
   __syncthreads();
   if (ln == 0) {
-    float* out = a.partial + (long long)g * a.ny * N + (long long)y * N;
+    const bool direct = a.ngroups == 1 && a.grad != nullptr;
+    float* out = direct ? a.grad + (long long)y * N : a.partial + (long long)g * a.ny * N + (long long)y * N;
 #pragma unroll
     for (int c = 0; c < E; ++c) {
       float s = 0.f;
       for (int l = 0; l < LINES; ++l) s += red[l * N + t + c * T];
-      out[t + c * T] = s;
+      out[t + c * T] = (direct && a.accumulate) ? out[t + c * T] + s : s;
     }
   }
 }

@@ -249,22 +249,6 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
 #pragma unroll
       for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
     }
-  } else if constexpr (MUL == RM_PHASE) {
-    if (ok) {
-      const float* vr = a.V + (long long)(b / a.t_group) * a.t_bstride + (long long)y * N;
-      // every potential value loaded before the first sincosf: its range
-      // reduction branches would otherwise put one memory round trip
-      // between consecutive elements
-      float ph[E];
-#pragma unroll
-      for (int c = 0; c < E; ++c) ph[c] = vr[t + c * T];
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        float sn, cs;
-        sincosf(a.sigma * ph[c], &sn, &cs);
-        v[c] = cscale(cmul(v[c], make_float2(cs, sn)), a.scale);
-      }
-    }
   } else {
     if (a.scale != 1.0f) {
 #pragma unroll
@@ -911,20 +895,15 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   else if (mode == ROW_FWD && mul == RM_NONE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_NONE, PRUNE_OUT);
   else if (mode == ROW_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_FWD, RM_T, 0);
   else if (mode == ROW_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_T, PRUNE_OUT);
-  else if (mode == ROW_FWD && mul == RM_PHASE && fl == 0) MSL_ROW(ROW_FWD, RM_PHASE, 0);
-  else if (mode == ROW_FWD && mul == RM_PHASE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_PHASE, PRUNE_OUT);
   else if (mode == ROW_INV && mul == RM_NONE && fl == 0) MSL_ROW(ROW_INV, RM_NONE, 0);
   else if (mode == ROW_INV && mul == RM_NONE && fl == PRUNE_IN) MSL_ROW(ROW_INV, RM_NONE, PRUNE_IN);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_INV_MUL_FWD, RM_T, 0);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_ROW(ROW_INV_MUL_FWD, RM_T, PRUNE_OUT);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT))
     MSL_ROW(ROW_INV_MUL_FWD, RM_T, PRUNE_IN | PRUNE_OUT);
-  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == 0) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, 0);
-  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == (PRUNE_IN | PRUNE_OUT))
-    MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, PRUNE_IN | PRUNE_OUT);
-  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == STORE_AUX) MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, STORE_AUX);
-  else if (mode == ROW_INV_MUL_FWD && mul == RM_PHASE && fl == (PRUNE_IN | PRUNE_OUT | STORE_AUX))
-    MSL_ROW(ROW_INV_MUL_FWD, RM_PHASE, PRUNE_IN | PRUNE_OUT | STORE_AUX);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == STORE_AUX) MSL_ROW(ROW_INV_MUL_FWD, RM_T, STORE_AUX);
+  else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT | STORE_AUX))
+    MSL_ROW(ROW_INV_MUL_FWD, RM_T, PRUNE_IN | PRUNE_OUT | STORE_AUX);
   else if (mode == ROW_INV_FD_FWD && mul == RM_T && fl == (PRUNE_IN | PRUNE_OUT))
     MSL_ROW(ROW_INV_FD_FWD, RM_T, PRUNE_IN | PRUNE_OUT);
   else if (mode == ROW_INV_FD_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_INV_FD_FWD, RM_T, 0);
@@ -1076,8 +1055,25 @@ bool prop_rows_quarter(const float* hky, int ny, float kmax) {
   return !(std::sqrt(kxv * kxv + kyv * kyv) < kmax);
 }
 
+// Column launches put the waves on grid.y, at most 65535 per launch: longer
+// batches run as consecutive launches over slices of the batch (the detector
+// pass, whose partial sums are indexed by wave, is refused instead).
+constexpr int kMaxGridY = 65535;
+
 hipError_t launch_col_v(int mode, int mul, const ColArgs& a, hipStream_t s, int var) {
   if (needs_prop_rows(mode, mul) && !a.prop_rows) return hipErrorInvalidValue;
+  if (a.nbatch > kMaxGridY) {
+    if (mode == COL_FWD_DETECT) return hipErrorInvalidValue;
+    for (long long b0 = 0; b0 < a.nbatch; b0 += kMaxGridY) {
+      ColArgs c = a;
+      c.nbatch = (int)std::min<long long>(kMaxGridY, a.nbatch - b0);
+      c.src = a.src + b0 * a.src_bstride;
+      c.dst = a.dst + b0 * a.dst_bstride;
+      const hipError_t e = launch_col_v(mode, mul, c, s, var);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   // the inverse transform after the propagator skips rows known to be zero
   if (mode == COL_FWD_MUL_INV && a.zb_rows) {
     if (mul == CM_PROP) mul = CM_PROP_ZB;

@@ -250,7 +250,17 @@ __device__ __forceinline__ float sample_z_rs(float zf, int m) {
   return zf - (float)m / 10.0f;
 }
 
-constexpr int PT = 16;      // pixel tile edge
+constexpr int PT = 16;      // pixel tile edge of the grouped launches (serving)
+// pixel tiles of 256 threads selectable per handle (PotArgs::tile, chosen by
+// msl_autotune_potential): 16x16, 32x8, 64x4, 8x32.  Every shape sums each
+// pixel's atoms in the same (slab, bin row, bin, CSR) order, so all give the
+// same bits; they differ in the bin window each workgroup stages.
+template <int V> struct PotTile;
+template <> struct PotTile<0> { static constexpr int X = 16, Y = 16; };
+template <> struct PotTile<1> { static constexpr int X = 32, Y = 8; };
+template <> struct PotTile<2> { static constexpr int X = 64, Y = 4; };
+template <> struct PotTile<3> { static constexpr int X = 8, Y = 32; };
+constexpr int kPotTiles = 4;
 constexpr int ACAP = 1024;  // atoms staged in LDS per chunk
 
 // binnedAtomPKernel3 (U.cu:582-653), binnedAtomPKernelConventional3
@@ -265,16 +275,18 @@ struct PotArgsG {
 };
 static_assert(sizeof(PotArgsG) == sizeof(PotArgs), "layouts differ");
 
-// pixel tile (tx, ty) of slice `slice` (relative to first_slice); 256 threads
-template <class A>
+// pixel tile (tx, ty) of TX x TY pixels (TX TY = 256 threads) of slice
+// `slice` (relative to first_slice)
+template <int TX, int TY, class A>
 __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int slice) {
+  static_assert(TX * TY == 256, "one thread per pixel of a 256-thread workgroup");
   __shared__ float zt[103 * 24];
   __shared__ float sx[ACAP], sy[ACAP], sz[ACAP];
   __shared__ int sZ[ACAP];
   __shared__ int rs[64], rl[65];
   const int tid = threadIdx.x;
   for (int i = tid; i < 103 * 24; i += 256) zt[i] = a.ztab[i];
-  const int ix = tx * PT + (tid % PT), iy = ty * PT + (tid / PT);
+  const int ix = tx * TX + (tid % TX), iy = ty * TY + (tid / TX);
   float zf, h;
   {
 #pragma clang fp contract(off)
@@ -304,8 +316,8 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
   const float px = pix_pos(ix, a.ps, offx), py = pix_pos(iy, a.ps, offy);
   // bin window of the tile: every bin that may hold an atom within sqrt(5)
   const float wx = (a.MaxX - a.MinX) / 50.f, wy = (a.MaxY - a.MinY) / 50.f, reach = 2.2360680f;
-  const float tx0 = offx + (float)(tx * PT) * a.ps, tx1 = offx + (float)(tx * PT + PT - 1) * a.ps;
-  const float ty0 = offy + (float)(ty * PT) * a.ps, ty1 = offy + (float)(ty * PT + PT - 1) * a.ps;
+  const float tx0 = offx + (float)(tx * TX) * a.ps, tx1 = offx + (float)(tx * TX + TX - 1) * a.ps;
+  const float ty0 = offy + (float)(ty * TY) * a.ps, ty1 = offy + (float)(ty * TY + TY - 1) * a.ps;
   const int i0 = max(0, (int)floorf((tx0 - reach) / wx) - 1), i1 = min(49, (int)floorf((tx1 + reach) / wx) + 1);
   const int j0 = max(0, (int)floorf((ty0 - reach) / wy) - 1), j1 = min(49, (int)floorf((ty1 + reach) / wy) + 1);
   const float h20 = a.dz / 20.0f;
@@ -315,15 +327,22 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
     for (int jb = j0; jb <= j1; jb += 64) {
       const int nr = min(64, j1 - jb + 1);
       if (i0 > i1) break;
-      if (tid < nr) {
-        const int bb = k * 2500 + (jb + tid) * 50;
-        rs[tid] = a.start[bb + i0];
-        rl[tid + 1] = a.start[bb + i1 + 1] - a.start[bb + i0];
-      }
-      __syncthreads();
-      if (tid == 0) {
-        rl[0] = 0;
-        for (int r = 0; r < nr; ++r) rl[r + 1] += rl[r];
+      if (tid < 64) {
+        // the first wave loads the window's bin-row runs and scans their
+        // lengths (inclusive, shuffles across the 64 lanes)
+        int len = 0;
+        if (tid < nr) {
+          const int bb = k * 2500 + (jb + tid) * 50;
+          const int r0 = a.start[bb + i0];
+          rs[tid] = r0;
+          len = a.start[bb + i1 + 1] - r0;
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+          const int v = __shfl_up(len, d, 64);
+          if (tid >= d) len += v;
+        }
+        if (tid < nr) rl[tid + 1] = len;
+        if (tid == 0) rl[0] = 0;
       }
       __syncthreads();
       const int tot = rl[nr];
@@ -331,8 +350,14 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
         const int cn = min(ACAP, tot - c0);
         for (int q = tid; q < cn; q += 256) {
           const int g = c0 + q;
-          int r = 0;
-          while (rl[r + 1] <= g) ++r;
+          // the run holding g: the largest r with rl[r] <= g (rl ascending)
+          int lo = 0, hi = nr - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rl[mid] <= g) lo = mid;
+            else hi = mid - 1;
+          }
+          const int r = lo;
           const int src = rs[r] + (g - rl[r]);
           sx[q] = a.bx[src];
           sy[q] = a.by[src];
@@ -346,7 +371,14 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
           const float* c = zt + (sZ[l] - 1) * 24;
           if (a.mode == 0) {
             const float zr = sz[l];
-            for (int m = 1; m <= 20; ++m) {
+            // samples m with |zf - m dz / 20 - zr| < sqrt(5 - rxy), widened by
+            // one sample on each side; inside that range each sample takes
+            // the exact FP32 cutoff test, so the set summed is the 20-sample
+            // loop's (the samples outside fail it by more than rounding)
+            const float hz = sqrtf(5.0f - rxy), u = 20.0f / a.dz;
+            const int mlo = max(1, (int)floorf((zf - zr - hz) * u) - 1);
+            const int mhi = min(20, (int)ceilf((zf - zr + hz) * u) + 1);
+            for (int m = mlo; m <= mhi; ++m) {
               const float r2 = dist2_z(rxy, sample_z(zf, m, a.dz), zr);
               if (r2 < 5.0f) sum += h20 * vatom_dev(c, r2);
             }
@@ -381,7 +413,10 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
   }
 }
 
-__global__ void __launch_bounds__(256) k_potential(PotArgs a) { potential_tile(a, blockIdx.x, blockIdx.y, blockIdx.z); }
+template <int V>
+__global__ void __launch_bounds__(256) k_potential(PotArgs a) {
+  potential_tile<PotTile<V>::X, PotTile<V>::Y>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 // slice `first` of njobs jobs in one launch: workgroup id -> (job, tile)
 // through the block offsets blk0[0..njobs] (ascending)
@@ -398,15 +433,26 @@ __global__ void __launch_bounds__(256) k_potential_group(const PotArgs* tab, con
   a.first_slice = first;
   const int tx = nblk(a.nx, PT);
   const long long local = id - blk0[lo];
-  potential_tile(a, (int)(local % tx), (int)(local / tx), 0);
+  potential_tile<PT, PT>(a, (int)(local % tx), (int)(local / tx), 0);
+}
+
+template <int V> static hipError_t potential_v(const PotArgs& a, hipStream_t s) {
+  dim3 grid(nblk(a.nx, PotTile<V>::X), nblk(a.ny, PotTile<V>::Y), a.nslices);
+  hipLaunchKernelGGL(k_potential<V>, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_potential(const PotArgs& a, hipStream_t s) {
   if (a.nslices <= 0) return hipSuccess;
-  dim3 grid(nblk(a.nx, PT), nblk(a.ny, PT), a.nslices);
-  hipLaunchKernelGGL(k_potential, grid, dim3(256), 0, s, a);
-  return hipGetLastError();
+  switch (a.tile) {
+    case 1: return potential_v<1>(a, s);
+    case 2: return potential_v<2>(a, s);
+    case 3: return potential_v<3>(a, s);
+    default: return potential_v<0>(a, s);
+  }
 }
+
+int potential_tile_variants() { return kPotTiles; }
 
 int potential_tiles(int nx, int ny) { return nblk(nx, PT) * nblk(ny, PT); }
 
@@ -444,8 +490,21 @@ __global__ void k_probe_kspace(ProbeArgs a) {
 }
 
 hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_probe_kspace, dim3(nblk(a.nx, 256), a.ny, a.nprobe), dim3(256), 0, s, a);
-  return hipGetLastError();
+  // probes on grid.z, at most 65535 per launch
+  constexpr int kMaxGridZ = 65535;
+  for (int b0 = 0; b0 < a.nprobe; b0 += kMaxGridZ) {
+    ProbeArgs c = a;
+    c.nprobe = a.nprobe - b0 < kMaxGridZ ? a.nprobe - b0 : kMaxGridZ;
+    c.out = a.out + (long long)b0 * a.bstride;
+    if (a.posx) {
+      c.posx = a.posx + b0;
+      c.posy = a.posy + b0;
+    }
+    hipLaunchKernelGGL(k_probe_kspace, dim3(nblk(a.nx, 256), a.ny, c.nprobe), dim3(256), 0, s, c);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // ============================================================== reductions
@@ -799,6 +858,21 @@ hipError_t launch_fd_update(const float2* grad, const float2* psim, float2* psip
   if (vper <= 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fd_update, dim3(nblk(n, 256)), dim3(256), 0, s, grad, psim, psip, psi, V, n, vper, sigma, wavel,
                      dz);
+  return hipGetLastError();
+}
+
+// ============================================================== transmission
+// t = exp(i sigma V) of one slice, computed once and read by the row pass
+// for every probe of the differentiable path
+__global__ void k_transmission(const float* V, float2* t, long long n, float sigma) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float sn, cs;
+  sincosf(sigma * V[i], &sn, &cs);
+  t[i] = make_float2(cs, sn);
+}
+hipError_t launch_transmission(const float* V, float2* t, long long n, float sigma, hipStream_t s) {
+  hipLaunchKernelGGL(k_transmission, dim3(nblk(n, 256)), dim3(256), 0, s, V, t, n, sigma);
   return hipGetLastError();
 }
 

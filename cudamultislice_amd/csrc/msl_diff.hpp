@@ -15,6 +15,10 @@ struct AdjArgs {
   int per_line;      // probes each line processes
   int ngroups;       // workgroups per row
   float* partial;    // [ngroups][ny][nx] gradient partial sums
+  // ngroups == 1: the gradient rows go straight to grad ([ny][nx]), added to
+  // what it holds when accumulate != 0; partial is unused
+  float* grad;
+  int accumulate;
   const float2* tw;  // twiddles for nx
   int prune;         // band-limited G / z (only x < N/4, x >= 3N/4 non-zero)
 };

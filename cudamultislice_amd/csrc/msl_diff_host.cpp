@@ -54,6 +54,7 @@ struct msl_diff {
   std::vector<float> hkx, hky;
   Buf<float> kx, ky, partial;
   Buf<float2> twx, twy, px, py, W, G, exitw, ckpt, seg;
+  Buf<float2> tslice;  // exp(i sigma V_i) of the slice being propagated
   Buf<int2> prow;  // rows of each column inside |k| < kmax
   const float* V = nullptr;
   const float2* psi0 = nullptr;
@@ -93,31 +94,42 @@ ColArgs cargs(msl_diff* d, const float2* src, float2* dst, float scale) {
 
 // probes per line and workgroups per row of the adjoint row pass for a
 // batch of nb probes
+// (up to 16 probes per line: one workgroup per row holds up to 64 probes
+// at 1024 and writes the gradient rows itself, with no partial maps; on the
+// 1024^2 x 100-slice x 64-probe workload fwd+bwd 94.1 ms against 99.5 ms with
+// 4 probes per line and partial maps)
 void adj_groups(int nx, int nb, int& per_line, int& ngroups) {
   const int lines = adj_lines(nx);
-  per_line = std::max(1, std::min(4, (nb + lines - 1) / lines));
+  per_line = std::max(1, std::min(16, (nb + lines - 1) / lines));
   ngroups = (nb + lines * per_line - 1) / (lines * per_line);
 }
 
-// The backward runs the adjoint of a segment chunk by chunk of probes: the
-// in-band half of a chunk's gradient waves (128 MiB) stays in the 256 MiB
-// Infinity Cache from one adjoint pass to the next while the stored waves
-// stream past it (1024^2 x 64 probes: 100.2 ms for chunks of 32 against
-// 102.1 ms unchunked; 16: 103.6, 8: 116.9 ms, launches of too few probes).
+// The backward can run the adjoint of a segment chunk by chunk of probes,
+// each chunk through all of the segment's slices (msl_diff_set_chunk).  With
+// the partial-map reduction chunks of 32 probes were faster (their gradient
+// waves stay in the Infinity Cache between passes: 100.2 vs 102.1 ms at
+// 1024^2 x 64 probes); with one workgroup per row writing the gradient
+// directly the whole batch in one chunk is (94.1 vs 95.7 ms), the default.
 int auto_chunk(int nx, int ny, int nprobe) {
-  const long long inband = (long long)nx * ny * 4;  // bytes of one probe's band-limited wave
-  const long long c = std::max(1LL, (128LL << 20) / inband);
-  return (int)std::min<long long>(nprobe, c);
+  (void)nx;
+  (void)ny;
+  return nprobe;
 }
 
 // slice i (1-based) forward: z_{i-1} (psi0 in real space for i = 1) -> z_i
 // at dst, through W (dst may be W: both passes work in place).  With wout
-// the row pass also stores w_i = IF_x(z_{i-1}) there.
+// the row pass also stores w_i = IF_x(z_{i-1}) there.  The slice's
+// transmission exp(i sigma V_i) is evaluated once into tslice and read by the
+// row pass for every probe (evaluated per line, the same sincosf would run
+// once per probe).
 void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, float2* wout, hipStream_t s) {
+  HC(launch_transmission(d->V + (size_t)(i - 1) * d->np(), d->tslice.p, d->np(), d->sigma, s));
   RowArgs r = rargs(d, zprev, d->W.p, i > 1, true);
-  r.V = d->V + (size_t)(i - 1) * d->np();
+  r.t = d->tslice.p;
+  r.t_group = d->B;
+  r.t_bstride = 0;
   if (i > 1) r.aux = wout;
-  HC(launch_row(i == 1 ? ROW_FWD : ROW_INV_MUL_FWD, RM_PHASE, r, s));
+  HC(launch_row(i == 1 ? ROW_FWD : ROW_INV_MUL_FWD, RM_T, r, s));
   HC(launch_col(COL_FWD_MUL_INV, CM_PROP, cargs(d, d->W.p, dst, d->nf * d->nf), s));
 }
 
@@ -201,6 +213,7 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     const size_t wave = (size_t)nprobe * nx * ny;
     d->W.alloc(wave);
     d->G.alloc(wave);
+    d->tslice.alloc((size_t)nx * ny);
     d->exitw.alloc(wave);
     if (d->S > 1) d->ckpt.alloc((size_t)(d->S - 1) * wave);
     d->seg.alloc((size_t)d->K * wave);
@@ -350,10 +363,14 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
           a.per_line = per_line;
           a.ngroups = ngroups;
           a.partial = d->partial.p;
+          a.grad = gradV + (size_t)(i - 1) * np;
+          a.accumulate = c0 > 0 ? 1 : 0;
           a.tw = d->twx.p;
           a.prune = d->prune_tiles > 0;
           HC(launch_row_adjoint(a, real_z, s));
-          HC(launch_sum_maps(d->partial.p, ngroups, np, gradV + (size_t)(i - 1) * np, c0 > 0 ? 1 : 0, s));
+          // one workgroup per row wrote the gradient rows itself
+          if (ngroups > 1)
+            HC(launch_sum_maps(d->partial.p, ngroups, np, gradV + (size_t)(i - 1) * np, c0 > 0 ? 1 : 0, s));
         }
       }
     }

@@ -28,6 +28,8 @@ using namespace mslh;
 namespace mslh {
 
 bool pow2_ok(int n) { return fft_size_supported(n); }
+// any side from 2 to 8192 (the generic transforms take every length)
+bool grid_ok(int n) { return n >= 2 && n <= 8192; }
 
 // complete symbol table, Z = 1..103
 const char* kSymbols[103] = {
@@ -173,8 +175,29 @@ void ensure_prop(msl_sim* s, float dz) {
   s->prop_dz = dz;
 }
 
+// generic transform of nbatch waves in place, then the scale
+void gen_fft2(msl_sim* s, float2* w, int nbatch, bool inverse, float scale) {
+  s->gscr.ensure((size_t)nbatch * s->npix());
+  HIPC(launch_gen_fft2(w, s->gscr.p, s->resX, s->resY, nbatch, inverse ? 1 : 0, s->twx.p, s->twy.p, s->stream));
+  if (scale != 1.0f)
+    HIPC(launch_gen_kmul(w, s->resX, s->resY, nbatch, CM_SCALE, nullptr, nullptr, nullptr, nullptr, 0.f, scale,
+                         s->stream));
+}
+// w *= (k-space multiplier mul) * scale, one wave (generic grids)
+void gen_kmul(msl_sim* s, float2* w, int mul, float scale) {
+  HIPC(launch_gen_kmul(w, s->resX, s->resY, 1, mul, s->kx.p, s->ky.p, s->px.p, s->py.p, s->kmax, scale, s->stream));
+}
+void copy_wave(msl_sim* s, float2* dst, const float2* src) {
+  if (dst != src)
+    HIPC(hipMemcpyAsync(dst, src, s->npix() * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+}
+
 // in-place unnormalised 2D transforms of nbatch waves
 void fft2(msl_sim* s, float2* w, int nbatch, bool inverse, float scale = 1.0f) {
+  if (s->generic) {
+    gen_fft2(s, w, nbatch, inverse, scale);
+    return;
+  }
   const long long bs = (long long)s->npix();
   if (!inverse) {
     row(s, ROW_FWD, RM_NONE, row_args(s, w, w, nbatch, bs));
@@ -192,6 +215,13 @@ void fft2(msl_sim* s, float2* w, int nbatch, bool inverse, float scale = 1.0f) {
 // w <- IFFT(P(dz) FFT(w)) nf^2, the propagation of U.cu:1986-1993
 void propagate(msl_sim* s, float2* src, float2* dst, float dz) {
   ensure_prop(s, dz);
+  if (s->generic) {
+    copy_wave(s, dst, src);
+    gen_fft2(s, dst, 1, false, 1.0f);
+    gen_kmul(s, dst, CM_PROP, s->nf * s->nf);
+    gen_fft2(s, dst, 1, true, 1.0f);
+    return;
+  }
   const long long bs = (long long)s->npix();
   row(s, ROW_FWD, RM_NONE, row_args(s, src, dst, 1, bs));
   ColArgs c = col_args(s, dst, dst, 1, bs);
@@ -277,7 +307,9 @@ void bin_atoms(msl_sim* s, const float* dx, const float* dy, const float* dz) {
 
 void calc_vars(msl_sim* s, float ps, float df, int rx, int ry, float sizeZ, float ox, float oy, bool stem) {
   require(s->have_micro, MSL_ERR_STATE, "ApplyMicroscopeParameters first");
-  require(pow2_ok(rx) && pow2_ok(ry), MSL_ERR_UNSUPPORTED, "resolution must be a power of two in 16..2048");
+  require(grid_ok(rx) && grid_ok(ry), MSL_ERR_UNSUPPORTED, "resolution must be 2..8192 per side");
+  // powers of two 16..2048 run the fused LDS passes, other sizes the generic ones
+  s->generic = !(pow2_ok(rx) && pow2_ok(ry));
   require(ps > 0, MSL_ERR_ARG, "pixel scale must be positive");
   s->resX = rx; s->resY = ry; s->df = df; s->ps = ps;
   s->offx = stem ? 0.f : ox; s->offy = stem ? 0.f : oy;
@@ -375,6 +407,7 @@ PotArgs pot_args(msl_sim* s, int mode, int first, int nslices, float dz, float z
   p.dz = dz; p.first_slice = first; p.nslices = nslices; p.zplane = zplane;
   p.loadslicesz = s->loadslicesz; p.sigma = s->sigma2; p.mode = mode; p.out_kind = out_kind;
   p.physical = (s->compat & MSL_COMPAT_PHYSICAL) ? 1 : 0;
+  p.tile = s->pot_tile;
   p.out = out; p.out_sstride = sstride;
   return p;
 }
@@ -452,6 +485,14 @@ struct KTimer {
 // annular |Psi_unitary|^2 of one real-space wave (copied into scratch)
 float adf_of(msl_sim* s, const float2* w, float inner, float outer) {
   const long long bs = (long long)s->npix();
+  if (s->generic) {
+    copy_wave(s, s->B.p, w);
+    gen_fft2(s, s->B.p, 1, false, 1.0f);
+    s->gpart.ensure(gen_annulus_blocks(bs));
+    HIPC(launch_gen_annulus(s->B.p, s->resX, s->resY, s->kx.p, s->ky.p, inner * 1e-3f / s->wavel,
+                            outer * 1e-3f / s->wavel, s->nf * s->nf, s->gpart.p, s->red.p + 2048, s->stream));
+    return download_scalar(s, s->red.p + 2048);
+  }
   row(s, ROW_FWD, RM_NONE, row_args(s, w, s->B.p, 1, bs));
   ColArgs c = col_args(s, s->B.p, nullptr, 1, bs);
   c.kin = inner * 1e-3f / s->wavel;
@@ -462,6 +503,35 @@ float adf_of(msl_sim* s, const float2* w, float inner, float outer) {
   const int ntiles = s->resX / col_tile(s->resY);
   HIPC(launch_reduce_rows(s->red.p, 1, ntiles, s->red.p + 2048, 0, s->stream));
   return download_scalar(s, s->red.p + 2048);
+}
+
+// the finite-difference step of U.cu:2050-2083 on the fused passes (grids of
+// powers of two): band limit of V, Laplacian of psi, update, band limit of psi+
+void fd_step_passes(msl_sim* s) {
+  const long long bs = (long long)s->npix();
+  // band-limit V: FFT, |k| <= kmax, IFFT, * nf^2
+  row(s, ROW_FWD, RM_NONE, row_args(s, s->V.p, s->B.p, 1, bs));
+  ColArgs c = col_args(s, s->B.p, s->B.p, 1, bs);
+  c.scale = s->nf * s->nf;
+  col(s, COL_FWD_MUL_INV, CM_BAND, c);
+  row(s, ROW_INV, RM_NONE, row_args(s, s->B.p, s->V.p, 1, bs));
+  // spectral Laplacian: FFT, * nf (-4 pi^2 k^2), IFFT, * nf
+  row(s, ROW_FWD, RM_NONE, row_args(s, s->Psi.p, s->G.p, 1, bs));
+  c = col_args(s, s->G.p, s->G.p, 1, bs);
+  c.scale = s->nf * s->nf;
+  col(s, COL_FWD_MUL_INV, CM_GRAD, c);
+  row(s, ROW_INV, RM_NONE, row_args(s, s->G.p, s->G.p, 1, bs));
+  HIPC(launch_fd_update(s->G.p, s->PsiMinus.p, s->PsiPlus.p, s->Psi.p, s->V.p, bs, bs, s->sigma2, s->wavel, s->dzFD,
+                        s->stream));
+  // band-limit psi+
+  row(s, ROW_FWD, RM_NONE, row_args(s, s->PsiPlus.p, s->B.p, 1, bs));
+  c = col_args(s, s->B.p, s->B.p, 1, bs);
+  c.scale = s->nf * s->nf;
+  col(s, COL_FWD_MUL_INV, CM_BAND, c);
+  row(s, ROW_INV, RM_NONE, row_args(s, s->B.p, s->PsiPlus.p, 1, bs));
+  // rotate: psi- <- psi, psi <- psi+
+  std::swap(s->PsiMinus.p, s->Psi.p);
+  std::swap(s->Psi.p, s->PsiPlus.p);
 }
 
 }  // namespace mslh
@@ -736,18 +806,26 @@ msl_status msl_multislice_step_conv(msl_sim* s, int it) {
   potential(s, s->potmode, it, 1, s->conventionaldz, 0.f, 0, s->V.p, bs);
   ensure_prop(s, s->conventionaldz);
   const float2* src = (it == 1) ? s->PsiMinus.p : s->Psi.p;
-  const bool prune = s->prune_tiles > 0;
-  RowArgs r = row_args(s, src, s->PsiPlus.p, 1, bs);
-  r.t = s->V.p;
-  r.prune_out = prune;
-  row(s, ROW_FWD, RM_T, r);
-  ColArgs c = col_args(s, s->PsiPlus.p, s->PsiPlus.p, 1, bs);
-  c.scale = s->nf * s->nf;
-  c.tiles_lo = s->prune_tiles;
-  col(s, COL_FWD_MUL_INV, CM_PROP, c);
-  RowArgs ri = row_args(s, s->PsiPlus.p, s->Psi.p, 1, bs);
-  ri.prune_in = prune;
-  row(s, ROW_INV, RM_NONE, ri);
+  if (s->generic) {
+    HIPC(launch_gen_cmul(src, s->V.p, s->PsiPlus.p, bs, s->stream));
+    gen_fft2(s, s->PsiPlus.p, 1, false, 1.0f);
+    gen_kmul(s, s->PsiPlus.p, CM_PROP, s->nf * s->nf);
+    gen_fft2(s, s->PsiPlus.p, 1, true, 1.0f);
+    copy_wave(s, s->Psi.p, s->PsiPlus.p);
+  } else {
+    const bool prune = s->prune_tiles > 0;
+    RowArgs r = row_args(s, src, s->PsiPlus.p, 1, bs);
+    r.t = s->V.p;
+    r.prune_out = prune;
+    row(s, ROW_FWD, RM_T, r);
+    ColArgs c = col_args(s, s->PsiPlus.p, s->PsiPlus.p, 1, bs);
+    c.scale = s->nf * s->nf;
+    c.tiles_lo = s->prune_tiles;
+    col(s, COL_FWD_MUL_INV, CM_PROP, c);
+    RowArgs ri = row_args(s, s->PsiPlus.p, s->Psi.p, 1, bs);
+    ri.prune_in = prune;
+    row(s, ROW_INV, RM_NONE, ri);
+  }
   API_END(s)
 }
 
@@ -764,29 +842,24 @@ msl_status msl_multislice_step(msl_sim* s, int it) {
     zc = h - d;
   }
   potential(s, 2, 0, 1, 1.0f, zc, 1, s->V.p, bs);
-  // band-limit V: FFT, |k| <= kmax, IFFT, * nf^2
-  row(s, ROW_FWD, RM_NONE, row_args(s, s->V.p, s->B.p, 1, bs));
-  ColArgs c = col_args(s, s->B.p, s->B.p, 1, bs);
-  c.scale = s->nf * s->nf;
-  col(s, COL_FWD_MUL_INV, CM_BAND, c);
-  row(s, ROW_INV, RM_NONE, row_args(s, s->B.p, s->V.p, 1, bs));
-  // spectral Laplacian: FFT, * nf (-4 pi^2 k^2), IFFT, * nf
-  row(s, ROW_FWD, RM_NONE, row_args(s, s->Psi.p, s->G.p, 1, bs));
-  c = col_args(s, s->G.p, s->G.p, 1, bs);
-  c.scale = s->nf * s->nf;
-  col(s, COL_FWD_MUL_INV, CM_GRAD, c);
-  row(s, ROW_INV, RM_NONE, row_args(s, s->G.p, s->G.p, 1, bs));
-  HIPC(launch_fd_update(s->G.p, s->PsiMinus.p, s->PsiPlus.p, s->Psi.p, s->V.p, bs, bs, s->sigma2, s->wavel, s->dzFD,
-                        s->stream));
-  // band-limit psi+
-  row(s, ROW_FWD, RM_NONE, row_args(s, s->PsiPlus.p, s->B.p, 1, bs));
-  c = col_args(s, s->B.p, s->B.p, 1, bs);
-  c.scale = s->nf * s->nf;
-  col(s, COL_FWD_MUL_INV, CM_BAND, c);
-  row(s, ROW_INV, RM_NONE, row_args(s, s->B.p, s->PsiPlus.p, 1, bs));
-  // rotate: psi- <- psi, psi <- psi+
-  std::swap(s->PsiMinus.p, s->Psi.p);
-  std::swap(s->Psi.p, s->PsiPlus.p);
+  if (s->generic) {
+    gen_fft2(s, s->V.p, 1, false, 1.0f);
+    gen_kmul(s, s->V.p, CM_BAND, s->nf * s->nf);
+    gen_fft2(s, s->V.p, 1, true, 1.0f);
+    copy_wave(s, s->G.p, s->Psi.p);
+    gen_fft2(s, s->G.p, 1, false, 1.0f);
+    gen_kmul(s, s->G.p, CM_GRAD, s->nf * s->nf);
+    gen_fft2(s, s->G.p, 1, true, 1.0f);
+    HIPC(launch_fd_update(s->G.p, s->PsiMinus.p, s->PsiPlus.p, s->Psi.p, s->V.p, bs, bs, s->sigma2, s->wavel,
+                          s->dzFD, s->stream));
+    gen_fft2(s, s->PsiPlus.p, 1, false, 1.0f);
+    gen_kmul(s, s->PsiPlus.p, CM_BAND, s->nf * s->nf);
+    gen_fft2(s, s->PsiPlus.p, 1, true, 1.0f);
+    std::swap(s->PsiMinus.p, s->Psi.p);
+    std::swap(s->Psi.p, s->PsiPlus.p);
+  } else {
+    fd_step_passes(s);
+  }
   API_END(s)
 }
 
@@ -1136,6 +1209,63 @@ msl_status msl_potential_slice(msl_sim* s, int it, float dz, int real, float* ou
   API_END(s)
 }
 
+msl_status msl_set_potential_tile(msl_sim* s, int variant) {
+  if (!s) return MSL_ERR_ARG;
+  if (variant < 0 || variant >= potential_tile_variants()) return MSL_ERR_ARG;
+  s->pot_tile = variant;
+  return MSL_OK;
+}
+
+// Times every pixel-tile shape of the potential kernel on the handle's atoms
+// and grid (nslices slices of thickness dz, its potential mode), keeps the
+// fastest whose potentials equal the 16x16 tile's bit for bit.
+msl_status msl_autotune_potential(msl_sim* s, float dz, int nslices, int* best, float* ms_per_variant) {
+  API_BEGIN(s)
+  require(s->have_freqs, MSL_ERR_STATE, "PreCalculateFrequencies first");
+  require(s->binned, MSL_ERR_STATE, "UploadBinnedAtoms first");
+  require(dz > 0 && nslices >= 1, MSL_ERR_ARG, "bad slice set");
+  const int nv = potential_tile_variants();
+  const long long np = (long long)s->npix();
+  DBuf<float> d;
+  d.ensure((size_t)nslices * np);
+  std::vector<float> ref((size_t)nslices * np), got((size_t)nslices * np);
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  const int saved = s->pot_tile;
+  int pick = 0;
+  float tbest = 0.f;
+  try {
+    for (int v = 0; v < nv; ++v) {
+      s->pot_tile = v;
+      potential(s, s->potmode, 1, nslices, dz, 0.f, 2, d.p, np);  // warm-up
+      HIPC(hipEventRecord(e0, s->stream));
+      potential(s, s->potmode, 1, nslices, dz, 0.f, 2, d.p, np);
+      HIPC(hipEventRecord(e1, s->stream));
+      HIPC(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      d2h(s, v == 0 ? ref.data() : got.data(), d.p, ref.size() * sizeof(float));
+      const bool same = v == 0 || std::memcmp(ref.data(), got.data(), ref.size() * sizeof(float)) == 0;
+      if (ms_per_variant) ms_per_variant[v] = same ? ms : -1.f;
+      if (same && (v == 0 || ms < tbest)) {
+        pick = v;
+        tbest = ms;
+      }
+    }
+  } catch (...) {
+    s->pot_tile = saved;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->pot_tile = pick;
+  if (best) *best = pick;
+  API_END(s)
+}
+
 msl_status msl_set_wave(msl_sim* s, int which, const float* in) {
   API_BEGIN(s)
   require(s->have_waves && in, MSL_ERR_STATE, "no waves");
@@ -1276,6 +1406,7 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
   require(s->have_freqs, MSL_ERR_STATE, "STEMSetCalculationVariables and PreCalculateFrequencies first");
   require(s->have_atoms, MSL_ERR_STATE, "no atoms");
   require(d->dz > 0, MSL_ERR_ARG, "slice thickness must be positive");
+  require(!s->generic, MSL_ERR_UNSUPPORTED, "batched STEM needs power-of-two sides 16..2048");
   s->cancel = 0;
   const double t0 = now_ms();
   const int nx = s->resX, ny = s->resY;

@@ -63,6 +63,7 @@ struct msl_sim {
   std::string err;
   unsigned compat = MSL_COMPAT_REFERENCE_SYMBOLS;
   int potmode = MSL_POT_INTEGRATED;
+  int pot_tile = 0;  // pixel tile shape of the potential kernel (msl_autotune_potential)
   volatile int cancel = 0;
 
   // structure (U.cuh:88-99, 127-132)
@@ -107,6 +108,11 @@ struct msl_sim {
 
   // waves
   mslh::DBuf<float2> PsiMinus, Psi, PsiPlus, V, G, B;
+  // sides not powers of two: the generic transforms (k_generic.hip) and
+  // their scratch; the batched, serving and differentiable paths refuse
+  bool generic = false;
+  mslh::DBuf<float2> gscr;
+  mslh::DBuf<float> gpart;
   bool have_waves = false, fd_ready = false;
   mslh::DBuf<float> tds, image, red;
   // stochastic detector model (SimulateImage, U.cu:2345-2428)

@@ -15,9 +15,8 @@ enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_INV_MUL_FWD = 2, ROW_INV_FD_FWD = 3
 // (src), psi+ from it, psi- (fd_pm), psi (fd_p0) and V (t, real part, slot
 // indexing as RM_T), then FFT_x of psi+ into dst.
 // pointwise factor applied between the (optional) inverse and forward
-// transforms: none, a stored transmission t, or exp(i sigma V) from a real
-// potential
-enum RowMul { RM_NONE = 0, RM_T = 1, RM_PHASE = 2 };
+// transforms: none, or a stored transmission t
+enum RowMul { RM_NONE = 0, RM_T = 1 };
 
 // The launch structures are written once as field lists parameterised by a
 // pointer qualifier G: empty for the host-visible structs below, the global
@@ -31,7 +30,6 @@ enum RowMul { RM_NONE = 0, RM_T = 1, RM_PHASE = 2 };
   int nx, ny, nbatch;                                                                                    \
   /* RM_T: transmission of wave b at t + (b / t_group) * t_bstride, or at t + t_index[b] * t_bstride */ \
   const float2 G* t;                                                                                     \
-  const float G* V; /* RM_PHASE: potential, same indexing */                                             \
   long long t_bstride;                                                                                   \
   int t_group;                                                                                           \
   float sigma;                                                                                           \
@@ -106,6 +104,19 @@ void set_pass_variant(int kind, int n, int v);
 int num_pass_variants(int kind);
 hipError_t launch_row_v(int mode, int mul, const RowArgs& a, hipStream_t s, int var);
 hipError_t launch_col_v(int mode, int mul, const ColArgs& a, hipStream_t s, int var);
+// grids whose sides are not powers of two (k_generic.hip): unnormalised 2D
+// FFT of nbatch [ny][nx] waves in place (scratch of the same size; twx / twy
+// the tables exp(-2 pi i k / n)), k-space multipliers (ColMul, one wave
+// layout, scale included), elementwise product, and the annular |F|^2 sum
+// (partial: gen_annulus_blocks floats) into out[0]
+hipError_t launch_gen_fft2(float2* data, float2* scratch, int nx, int ny, int nbatch, int inverse,
+                           const float2* twx, const float2* twy, hipStream_t s);
+hipError_t launch_gen_kmul(float2* w, int nx, int ny, int nbatch, int mul, const float* kx, const float* ky,
+                           const float2* px, const float2* py, float kmax, float scale, hipStream_t s);
+hipError_t launch_gen_cmul(const float2* src, const float2* t, float2* dst, long long n, hipStream_t s);
+int gen_annulus_blocks(long long np);
+hipError_t launch_gen_annulus(const float2* F, int nx, int ny, const float* kx, const float* ky, float kin,
+                              float kout, float scale, float* partial, float* out, hipStream_t s);
 // deterministic pseudo-random fill: unit != 0 gives exp(i theta) values
 hipError_t launch_hash_fill(float2* w, long long n, unsigned seed, int unit, hipStream_t s);
 // streaming copy of `bytes` (multiple of 16)
@@ -176,6 +187,7 @@ hipError_t launch_bin(const BinArgs& a, hipStream_t s);
   float zplane;              /* modes 2, 3: plane position (single slice) */                             \
   int loadslicesz;                                                                                       \
   int physical;              /* modes 0, 1: compat=physical slice position and slab window */            \
+  int tile;                  /* pixel tile shape (potential_tile_variants(); 0: 16x16) */               \
   float sigma;                                                                                           \
   /* mode 0 3D-integrated slice, 1 projected, 2 FD plane, 3 real-space 1 A slab; */                     \
   /* out_kind 0 exp(i sigma V), 1 (V, 0), 2 V */                                                         \
@@ -190,6 +202,7 @@ hipError_t launch_potential(const PotArgs& a, hipStream_t s);
 // tab[j] (device) holds job j's arguments, blk0[0..njobs] (device) the first
 // workgroup of each job, job j owning potential_tiles(nx, ny) workgroups
 int potential_tiles(int nx, int ny);
+int potential_tile_variants();
 hipError_t launch_potential_group(const PotArgs* tab, const long long* blk0, int njobs, long long nblocks, int first,
                                   hipStream_t s);
 void kirkland_transform(const float* fparams, float* ztab);  // host: 103x12 -> per-Z constants
@@ -246,6 +259,8 @@ hipError_t launch_shot_noise(const float2* z, float2* out, long long n, float zs
                              unsigned long long seed, hipStream_t s);
 hipError_t launch_real_to_complex(const float* in, float2* out, long long n, hipStream_t s);
 hipError_t launch_intensity(const float2* w, float* out, long long n, float s, hipStream_t st);
+// t[i] = exp(i sigma V[i])
+hipError_t launch_transmission(const float* V, float2* t, long long n, float sigma, hipStream_t s);
 
 // TDS / CBED: tds[y][x] += |F[(y+ny/2)%ny][(x+nx/2)%nx]| (U.cu:2734-2766)
 hipError_t launch_tds_accum(const float2* F, float* tds, int nx, int ny, hipStream_t s);

@@ -633,6 +633,8 @@ void check_job(const msl_job& d, int i) {
   require(d.nslices >= 1, MSL_ERR_ARG, w + "nslices must be positive");
   require(d.natoms >= 1 && d.Z && d.x && d.y && d.z, MSL_ERR_ARG, w + "no atoms");
   require(d.nprobe >= 0, MSL_ERR_ARG, w + "negative probe count");
+  // one detector launch per job and batch holds its probes on grid.y
+  require(d.nprobe <= 65535, MSL_ERR_UNSUPPORTED, w + "more than 65535 probes in one job");
   if (d.nprobe > 0) {
     require(d.posx && d.posy && d.adf_out, MSL_ERR_ARG, w + "probe positions and adf_out required");
     for (int p = 0; p < d.nprobe; ++p)

@@ -4,16 +4,45 @@
 intensities |psi_n|^2 of a batch of probes propagated through per-slice
 potentials V (float32, [nslices, ny, nx], V*A); its backward returns dL/dV
 through the hand-written adjoint kernels of libmsl (waves checkpointed every
-`checkpoint_every` slices, or all kept when they fit).  torch supplies device memory and the stream;
-all arithmetic runs in libmsl.
+`checkpoint_every` slices, or all kept when they fit).  The engine is the
+torch extension class torch.classes.msl.DiffEngine (libmsl_torch.so, over
+the msl_diff_* C-ABI); torch supplies device memory and the stream, all
+arithmetic runs in libmsl.
 """
 from __future__ import annotations
 
-import ctypes as C
+import os
+import re
 
 import torch
 
 from ._lib import MslError, lib
+
+EXT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmsl_torch.so")
+_loaded = False
+
+
+def load_extension() -> None:
+    """Registers torch.classes.msl (raises if the extension is not built)."""
+    global _loaded
+    if _loaded:
+        return
+    if not os.path.exists(EXT_PATH):
+        raise RuntimeError(f"{EXT_PATH} missing: build it (make -C cudamultislice_amd/csrc)")
+    lib()  # libmsl.so first, bound to the HIP runtime torch already loaded
+    torch.ops.load_library(EXT_PATH)
+    _loaded = True
+
+
+def _call(f, *args):
+    """f(*args), with the extension's failing msl status raised as MslError"""
+    try:
+        return f(*args)
+    except RuntimeError as e:
+        m = re.search(r"msl status (\d+)", str(e))
+        if m:
+            raise MslError(int(m.group(1)), str(e)) from None
+        raise
 
 
 class DiffEngine:
@@ -21,67 +50,42 @@ class DiffEngine:
         """checkpoint_every: K > 0 keeps every K-th wave and recomputes the rest
         in the backward; 0 keeps every slice's wave; -1 (default) the longest
         segment whose waves fit in half the free device memory"""
+        load_extension()
         self.shape = (nprobe, ny, nx)
         self.nslices = nslices
         self.device = device
-        h = C.c_void_p()
-        st = lib().msl_diff_create(C.byref(h), device, nx, ny, ps, kV, dz, nslices, nprobe, checkpoint_every)
-        if st != 0:
-            raise MslError(st, "msl_diff_create")
-        self._h = h
+        self._e = _call(torch.classes.msl.DiffEngine, device, nx, ny, float(ps), float(kV), float(dz), nslices,
+                        nprobe, checkpoint_every)
         self._gen = 0
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().msl_diff_destroy(self._h)
-            self._h = None
+        """releases the engine's device memory (once no graph holds it)"""
+        self._e = None
 
     def __del__(self):
         self.close()
 
     def info(self):
-        wl, sg, km, ws = C.c_float(), C.c_float(), C.c_float(), C.c_longlong()
-        lib().msl_diff_info(self._h, C.byref(wl), C.byref(sg), C.byref(km), C.byref(ws))
-        return {"wavelength": wl.value, "sigma": sg.value, "kmax": km.value, "workspace_bytes": ws.value}
+        wl, sg, km, ws = _call(self._e.info)
+        return {"wavelength": wl, "sigma": sg, "kmax": km, "workspace_bytes": int(ws)}
 
     def set_chunk(self, probes: int) -> None:
         """probes per adjoint chunk of the backward (0: automatic)"""
-        self._check(lib().msl_diff_set_chunk(self._h, int(probes)))
-
-    def _check(self, st):
-        if st != 0:
-            raise MslError(st, lib().msl_diff_last_error(self._h).decode())
-
-    def _validate(self, V, psi0):
-        B, ny, nx = self.shape
-        if V.dtype != torch.float32 or tuple(V.shape) != (self.nslices, ny, nx) or not V.is_contiguous():
-            raise ValueError(f"V must be contiguous float32 {(self.nslices, ny, nx)}")
-        if psi0.dtype != torch.complex64 or tuple(psi0.shape) != (B, ny, nx) or not psi0.is_contiguous():
-            raise ValueError(f"psi0 must be contiguous complex64 {(B, ny, nx)}")
-        if V.device.type != "cuda" or psi0.device.type != "cuda":
-            raise ValueError("V and psi0 must be on the GPU")
+        _call(self._e.set_chunk, int(probes))
 
     def forward(self, V, psi0, keep=True):
         """intensities |psi_n|^2; keep=False stores nothing for a backward
         (inference), after which backward raises until the next forward"""
-        self._validate(V, psi0)
-        out = torch.empty(self.shape, dtype=torch.float32, device=V.device)
         s = torch.cuda.current_stream(V.device).cuda_stream
-        f = lib().msl_diff_forward if keep else lib().msl_diff_forward_only
-        self._check(f(self._h, C.c_void_p(V.data_ptr()), C.c_void_p(psi0.data_ptr()), C.c_void_p(out.data_ptr()),
-                      C.c_void_p(s)))
+        out = _call(self._e.forward, V, psi0, bool(keep), s)
         self._gen += 1
         return out
 
     def backward(self, dLdI, gen=None):
         if gen is not None and gen != self._gen:
             raise RuntimeError("DiffEngine: another forward ran since this graph was built")
-        dLdI = dLdI.contiguous().to(torch.float32)
-        g = torch.empty((self.nslices,) + self.shape[1:], dtype=torch.float32, device=dLdI.device)
         s = torch.cuda.current_stream(dLdI.device).cuda_stream
-        self._check(lib().msl_diff_backward(self._h, C.c_void_p(dLdI.data_ptr()), C.c_void_p(g.data_ptr()),
-                                            C.c_void_p(s)))
-        return g
+        return _call(self._e.backward, dLdI, s)
 
 
 class MultisliceIntensity(torch.autograd.Function):

@@ -232,6 +232,17 @@ class MultisliceSimulation:
         self._c(lib().msl_potential_slice(self._h, it, dz, 1 if real else 0, _p(o)))
         return o
 
+    def autotune_potential(self, dz: float, nslices: int = 4):
+        """times the potential kernel's pixel-tile shapes on this handle's
+        atoms and grid, keeps the fastest; returns (best, ms per shape)"""
+        best = C.c_int(0)
+        ms = np.zeros(4, np.float32)
+        self._c(lib().msl_autotune_potential(self._h, dz, int(nslices), C.byref(best), _p(ms)))
+        return best.value, ms.tolist()
+
+    def set_potential_tile(self, variant: int):
+        self._c(lib().msl_set_potential_tile(self._h, int(variant)))
+
     def set_wave(self, w, which=0):
         w = np.ascontiguousarray(w, np.complex64)
         self._c(lib().msl_set_wave(self._h, which, _p(w)))

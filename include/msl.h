@@ -258,9 +258,9 @@ msl_status msl_diff_forward(msl_diff* d, const float* V, const void* psi0, float
 msl_status msl_diff_forward_only(msl_diff* d, const float* V, const void* psi0, float* intensity, void* stream);
 msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void* stream);
 msl_status msl_diff_info(const msl_diff* d, float* wavelength, float* sigma, float* kmax, long long* workspace_bytes);
-/* probes per adjoint chunk of the backward (0: automatic, the probes whose
- * band-limited gradient waves fill 128 MiB); every chunk runs through a whole
- * segment before the next, and gradient rows are summed in chunk order */
+/* probes per adjoint chunk of the backward (0: automatic, all probes in one
+ * chunk); every chunk runs through a whole segment before the next, and
+ * gradient rows are summed in chunk order */
 msl_status msl_diff_set_chunk(msl_diff* d, int probes);
 
 /* ---- pass layouts and autotuning (no counterpart in the reference) ----
@@ -345,6 +345,14 @@ msl_status msl_server_run(msl_server* srv, const msl_serve_params* prm, const ms
 /* batched 2D FFT of nbatch [ny][nx] complex64 device arrays, in place,
  * unnormalised; inverse != 0 for exp(+2 pi i) */
 msl_status msl_fft2_device(int nx, int ny, int nbatch, void* data, int inverse, void* stream);
+/* pixel tile of the potential kernel: 0 16x16 (default), 1 32x8, 2 64x4,
+ * 3 8x32 threads; all give the same bits.  msl_autotune_potential times
+ * each on the handle's binned atoms and grid (nslices slices of thickness
+ * dz from the top) and keeps the fastest whose potentials equal the 16x16
+ * tile's; ms_per_variant (optional, 4 floats) gets the times (-1 where
+ * rejected). */
+msl_status msl_set_potential_tile(msl_sim* sim, int variant);
+msl_status msl_autotune_potential(msl_sim* sim, float dz, int nslices, int* best, float* ms_per_variant);
 /* bin / potential on device arrays held by a handle (debug access) */
 msl_status msl_get_binned(const msl_sim* sim, int* start, float* x, float* y, float* z, int* Z);
 msl_status msl_get_bins(const msl_sim* sim, int* keys);

@@ -736,8 +736,8 @@ def test_invalid_calls_return_status_codes(fp):
         g.GetParameterisation()
         g.UploadBinnedAtoms()
         g.ApplyMicroscopeParameters(300.0)
-        assert code(g.SetCalculationVariables, 0.2, 0.0, 1000, 1000, 1, 1, 8.0) == UNSUPPORTED  # not 2^k
-        assert code(g.SetCalculationVariables, 0.2, 0.0, 4096, 4096, 1, 1, 8.0) == UNSUPPORTED  # > 2048
+        assert code(g.SetCalculationVariables, 0.2, 0.0, 10000, 64, 1, 1, 8.0) == UNSUPPORTED  # > 8192
+        assert code(g.SetCalculationVariables, 0.2, 0.0, 64, 1, 1, 1, 8.0) == UNSUPPORTED      # < 2
         g.SetCalculationVariables(0.2, 0.0, 64, 64, 1, 1, 8.0)
         g.PreCalculateFrequencies()
         assert code(g.MultisliceStepConv, 1) == STATE                          # before InitialiseWavefunctions

@@ -64,3 +64,20 @@ def test_tiff_writer_round_trips_through_an_independent_reader(tmp_path):
     L = _lib.lib()
     assert L.msl_write_tiff(p.encode(), 0, 4, a.ctypes.data_as(C.c_void_p)) == 1
     assert L.msl_write_tiff(str(tmp_path / "no" / "such.tiff").encode(), 4, 4, a.ctypes.data_as(C.c_void_p)) == 3
+
+
+@needs_lib
+def test_torch_extension_registers_the_engine_class():
+    """libmsl_torch.so registers torch.classes.msl.DiffEngine with the
+    forward / backward / set_chunk / info methods the autograd binding calls;
+    without a GPU its constructor reports the HIP failure as an msl status."""
+    import torch
+    from cudamultislice_amd import diff
+    diff.load_extension()
+    cls = torch.classes.msl.DiffEngine
+    assert cls is not None
+    if not torch.cuda.is_available():
+        from cudamultislice_amd._lib import MslError
+        with pytest.raises(MslError) as e:
+            diff.DiffEngine(64, 64, 0.25, 300.0, 2.0, 4, 2, checkpoint_every=2)
+        assert e.value.code == 4
