@@ -92,6 +92,7 @@ _SIGS = {
     "msl_fft2_device": ([_i, _i, _i, _vp, _i, _vp], _i),
     "msl_set_potential_tile": ([_vp, _i], _i),
     "msl_autotune_potential": ([_vp, _f, _i, C.POINTER(_i), _vp], _i),
+    "msl_fft2_device_noalloc": ([_i, _i, _i, _vp, _i, _vp], _i),
     "msl_set_pass_variant": ([_i, _i, _i], _i), "msl_get_pass_variant": ([_i, _i, C.POINTER(_i)], _i),
     "msl_bench_pass": ([_i, _i, _i, _i, _i, C.POINTER(_f), C.POINTER(_f)], _i),
     "msl_autotune_passes": ([_i, _i, _i, C.POINTER(_i), C.POINTER(_i), _vp], _i),

@@ -345,6 +345,15 @@ msl_status msl_server_run(msl_server* srv, const msl_serve_params* prm, const ms
 /* batched 2D FFT of nbatch [ny][nx] complex64 device arrays, in place,
  * unnormalised; inverse != 0 for exp(+2 pi i) */
 msl_status msl_fft2_device(int nx, int ny, int nbatch, void* data, int inverse, void* stream);
+/* msl_fft2_device without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  No counterpart in the reference.  The twiddle tables that
+ * msl_fft2_device allocates per call live in one static 32 KiB device array
+ * per GPU instead: nx, ny <= 2048 (MSL_ERR_UNSUPPORTED beyond), nbatch <=
+ * 65535.  Work is queued on `stream` and not waited for.  Every call on a GPU
+ * rewrites that array: calls that overlap on one GPU race and must be
+ * serialised by the caller. */
+msl_status msl_fft2_device_noalloc(int nx, int ny, int nbatch, void* data, int inverse, void* stream);
 /* pixel tile of the potential kernel: 0 16x16 (default), 1 32x8, 2 64x4,
  * 3 8x32 threads; all give the same bits.  msl_autotune_potential times
  * each on the handle's binned atoms and grid (nslices slices of thickness
