@@ -885,3 +885,28 @@ def test_compat_modes_match_the_oracle_quirk_by_quirk(fp, physical):
     assert want.min() > 0
     assert np.max(np.abs(r["adf"] - want) / want) < 1e-4
     g.close()
+
+
+def test_potential_tile_shapes_give_the_same_bits_and_the_tuner_keeps_one(fp):
+    """The potential kernel's pixel tiles (16x16, 32x8, 64x4, 8x32) stage
+    different bin windows but sum each pixel's atoms in the same order: the
+    same bits for every shape, equal to the oracle within 2e-5; the tuner
+    keeps the fastest shape."""
+    Z, x, y, z = S.srtio3(3, 3, 4)
+    g, o = _both(Z, x, y, z, fp)
+    g.UploadBinnedAtoms()
+    o.bin()
+    _setup(g, o, 300.0, 0.15, 96, 20.0, off=(2.0, 2.0))
+    outs = []
+    for v in range(4):
+        g.set_potential_tile(v)
+        outs.append(np.stack([g.potential_slice(it, 2.0, real=True) for it in (1, 3, 5)]))
+    for v in range(1, 4):
+        assert np.array_equal(outs[v], outs[0]), v
+    e = o.extents()
+    H = np.float32(e[4] - e[5])
+    want = o.potential(3, float(H - np.float32(6.0)), 2.0).real
+    assert np.max(np.abs(outs[0][1] - want)) < 2e-5 * np.max(want)
+    best, ms = g.autotune_potential(2.0, nslices=4)
+    assert best in range(4) and ms[best] > 0 and all(m > 0 for m in ms)
+    assert np.array_equal(np.stack([g.potential_slice(it, 2.0, real=True) for it in (1, 3, 5)]), outs[0])

@@ -269,23 +269,6 @@ def test_diff_amorphous_carbon_1024_100_slices_matches_fp64_adjoint(fp):
 
 
 # ---------------------------------------------------------------- serving mix
-def _mix_sample(jobs):
-    """14 of 256 jobs (5.5 %): at grids 256, 512 and 1024 the cheapest job
-    (pixel-slices times probes, finite differences counted twice) of each
-    (method, plane wave / probes) class, and at 2048 the cheapest
-    plane-wave job of each method."""
-    from cudamultislice_amd.serve import FD
-    pick, cost = {}, {}
-    for k, j in enumerate(jobs):
-        if j.n == 2048 and j.nprobe > 0:
-            continue
-        key = (j.n, j.method, j.nprobe > 0)
-        c = j.n * j.n * j.nslices * max(1, j.nprobe) * (2 if j.method == FD else 1)
-        if key not in pick or c < cost[key]:
-            pick[key], cost[key] = k, c
-    return sorted(pick.values())
-
-
 @pytest.mark.timeout(1200)
 def test_serving_mix_256_jobs_with_an_oracle_sample(fp):
     """The uneven serving mix: 256 seeded jobs (grids 256-2048, 20-200
@@ -305,7 +288,7 @@ def test_serving_mix_256_jobs_with_an_oracle_sample(fp):
     for j in jobs:
         r = j.result.get("adf", j.result.get("exit"))
         assert r is not None and np.all(np.isfinite(r))
-    sample = _mix_sample(jobs)
+    sample = O.serving_sample(jobs)
     assert len(sample) >= 0.05 * len(jobs)
     assert {jobs[k].n for k in sample} == {256, 512, 1024, 2048}
     worst = {}
