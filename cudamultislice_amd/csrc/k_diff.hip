@@ -19,10 +19,12 @@ namespace msl {
 
 // One line per T threads; lines never straddle a wavefront when T <= 64, so
 // the FFT exchanges synchronise the wave only (padded LDS, as the row pass).
-template <int N> struct AdjCfg {
+// At 1024 the pass holds 216 VGPRs (two waves per SIMD): two four-line
+// workgroups per CU, and a 1024-row grid fills the 512 slots twice.
+template <int N, bool REAL_Z> struct AdjCfg {
   static constexpr int T = Plan<N>::T;
-  static constexpr int L = (256 / T) > 0 ? 256 / T : 1;
   static constexpr bool WAVE = T <= 64;
+  static constexpr int L = (256 / T) > 0 ? 256 / T : 1;
   static constexpr int SH = WAVE ? 4 : 0;
   static constexpr int LS = LdsLine<SH>::span(N) + 1;
 };
@@ -35,38 +37,26 @@ template <int E> __device__ __forceinline__ constexpr bool band_block(int c) {
 // REAL_Z: Z holds w_i = psi_{i-1} in real space (stored by the forward row
 // pass, or psi0), so only G is transformed; otherwise Z holds z_{i-1} in the
 // (kx, y) domain and w_i = IF_x(z_{i-1}) is recomputed here.
-// The row's transmission exp(i sigma V) is computed once per workgroup into
-// LDS, shared by its LINES lines and every probe they process (each thread
-// evaluated sincosf for its 16 elements of every line before: 3204 -> 1832
-// VALU instructions at 1024, 220 -> 190 VGPRs with w_i loaded after the first
-// transform).  Asking the compiler for three waves per SIMD spills at 1024.
+// The row's transmission exp(i sigma V) comes from the forward pass (one
+// array per slice): every line of the workgroup processes row y, so each
+// thread keeps its E values in registers for all the probes it processes.
 template <int N, bool REAL_Z, bool PRUNE>
-__global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a) {
-  using Cfg = AdjCfg<N>;
+__global__ void __launch_bounds__((AdjCfg<N, REAL_Z>::L * Plan<N>::T)) k_row_adj(AdjArgs a) {
+  using Cfg = AdjCfg<N, REAL_Z>;
   using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
   using LL = LdsLine<Cfg::SH>;
   constexpr int T = Plan<N>::T, E = N / T, LINES = Cfg::L, LS = Cfg::LS, NT = LINES * T;
   static_assert(LS * 2 >= N, "the reduction reuses the line buffers");
   __shared__ float2 lds[LINES * LS];
   __shared__ float2 stw[N];  // twiddles staged in LDS
-  __shared__ float2 str[N];  // transmission of row y
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   const int y = blockIdx.x, g = blockIdx.y;
   for (int i = tid; i < N; i += NT) stw[i] = a.tw[i];
+  float2 tr[E];
   {
-    const float* vr = a.V + (long long)y * N;
-    // every potential value loaded before the first sincosf (whose range
-    // reduction branches would serialise the loads)
-    constexpr int K = N / NT > 0 ? N / NT : 1;
-    float v[K];
+    const float2* trow = a.t + (long long)y * N;
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = (tid + k * NT < N) ? vr[tid + k * NT] : 0.f;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      float sn, cs;
-      sincosf(a.sigma * v[k], &sn, &cs);
-      if (tid + k * NT < N) str[tid + k * NT] = make_float2(cs, sn);
-    }
+    for (int c = 0; c < E; ++c) tr[c] = trow[t + c * T];
   }
   __syncthreads();
   const LL L{lds + ln * LS, 1};
@@ -107,7 +97,7 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
       if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S, LL, ZIN>(w, t, stw, L);  // w = IF_x(z_{i-1})
 #pragma unroll
       for (int c = 0; c < E; ++c) {
-        const float2 gw = cmulc(gu[c], str[t + c * T]);  // conj(t) G_u
+        const float2 gw = cmulc(gu[c], tr[c]);  // conj(t) G_u
         acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
         gu[c] = gw;
       }
@@ -137,29 +127,29 @@ __global__ void __launch_bounds__(AdjCfg<N>::L * Plan<N>::T) k_row_adj(AdjArgs a
   }
 }
 
+
 template <int N> static hipError_t adj_n(const AdjArgs& a, bool real_z, hipStream_t s) {
-  constexpr int LINES = AdjCfg<N>::L, T = Plan<N>::T;
-  dim3 grid(a.ny, a.ngroups), block(LINES * T);
+  constexpr int T = Plan<N>::T;
+  dim3 grid(a.ny, a.ngroups);
   if (real_z) {
+    dim3 block(AdjCfg<N, true>::L * T);
     if (a.prune) hipLaunchKernelGGL((k_row_adj<N, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_row_adj<N, true, false>), grid, block, 0, s, a);
   } else {
+    dim3 block(AdjCfg<N, false>::L * T);
     if (a.prune) hipLaunchKernelGGL((k_row_adj<N, false, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((k_row_adj<N, false, false>), grid, block, 0, s, a);
   }
   return hipGetLastError();
 }
 
-int adj_lines(int nx) {
+int adj_lines(int nx, bool real_z) {
   switch (nx) {
-    case 16: return AdjCfg<16>::L;
-    case 32: return AdjCfg<32>::L;
-    case 64: return AdjCfg<64>::L;
-    case 128: return AdjCfg<128>::L;
-    case 256: return AdjCfg<256>::L;
-    case 512: return AdjCfg<512>::L;
-    case 1024: return AdjCfg<1024>::L;
-    case 2048: return AdjCfg<2048>::L;
+#define MSL_ADJ_LINES(n) \
+  case n: return real_z ? AdjCfg<n, true>::L : AdjCfg<n, false>::L;
+    MSL_ADJ_LINES(16) MSL_ADJ_LINES(32) MSL_ADJ_LINES(64) MSL_ADJ_LINES(128)
+    MSL_ADJ_LINES(256) MSL_ADJ_LINES(512) MSL_ADJ_LINES(1024) MSL_ADJ_LINES(2048)
+#undef MSL_ADJ_LINES
   }
   return 0;
 }

@@ -46,6 +46,7 @@ static int lg2(int n) {
 //         1 wave-local exchanges (lines never straddle a wave), LDS pad 1/16
 //         2 as 1 with the alternative plan (AltPlan: other radices and
 //           threads per line)
+//         3 as 1 with 512-thread workgroups (twice the lines)
 // column: 0 [row][col] interleaved LDS, workgroup barriers
 //         1 [col][row] LDS with pad 1/16, workgroup barriers
 //         2 as 0 with the alternative plan
@@ -73,7 +74,7 @@ void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 3 : 7; }
+int num_pass_variants(int kind) { return kind == 0 ? 4 : 7; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
@@ -138,7 +139,10 @@ template <int N, int RV> struct RowCfg {
   static constexpr int T = P::T;
   static constexpr bool WAVE = RV >= 1 && T <= 64;
   static constexpr int SH = RV >= 1 ? 4 : 0;
-  static constexpr int LINES = (256 / T) > 0 ? 256 / T : 1;
+  // variant 3: 512-thread workgroups (eight one-wave lines at 512 and 1024:
+  // two workgroups, sixteen waves per CU at 1024, where four-line
+  // workgroups of 43 KB of LDS fit three)
+  static constexpr int LINES = ((RV == 3 ? 512 : 256) / T) > 0 ? (RV == 3 ? 512 : 256) / T : 1;
   static constexpr int LS = LdsLine<SH>::span(N) + 1;
 };
 
@@ -914,6 +918,7 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
 
 template <int N>
 static hipError_t row_n(int mode, int mul, const RowArgs& a, hipStream_t s, int var) {
+  if (var == 3) return row_nv<N, 3>(mode, mul, a, s);
   if (var == 2) return row_nv<N, 2>(mode, mul, a, s);
   if (var == 1) return row_nv<N, 1>(mode, mul, a, s);
   return row_nv<N, 0>(mode, mul, a, s);

@@ -8,7 +8,7 @@ struct AdjArgs {
   const float2* G;   // gradient w.r.t. the row-pass output a_i, (kx, y) domain
   const float2* Z;   // real_z: w_i = psi_{i-1} in real space; else z_{i-1} in the (kx, y) domain
   float2* Gout;      // gradient w.r.t. z_{i-1} (may alias G)
-  const float* V;    // potential of slice i, [ny][nx]
+  const float2* t;   // transmission exp(i sigma V_i) of slice i, [ny][nx]
   float sigma;
   long long bstride; // elements between probes
   int nx, ny, nbatch;
@@ -23,7 +23,7 @@ struct AdjArgs {
   int prune;         // band-limited G / z (only x < N/4, x >= 3N/4 non-zero)
 };
 
-int adj_lines(int nx);
+int adj_lines(int nx, bool real_z);
 hipError_t launch_row_adjoint(const AdjArgs& a, bool real_z, hipStream_t s);
 
 }  // namespace msl

@@ -54,12 +54,12 @@ struct msl_diff {
   std::vector<float> hkx, hky;
   Buf<float> kx, ky, partial;
   Buf<float2> twx, twy, px, py, W, G, exitw, ckpt, seg;
-  Buf<float2> tslice;  // exp(i sigma V_i) of the slice being propagated
+  Buf<float2> tall;    // exp(i sigma V_i) of every slice, [ns][ny][nx], written by the forward
   Buf<int2> prow;  // rows of each column inside |k| < kmax
   const float* V = nullptr;
   const float2* psi0 = nullptr;
   bool have_forward = false;
-  int ngroups = 1, per_line = 1;
+  int ngroups = 1;      // partial gradient maps allocated (largest adjoint group count)
   int chunk = 1;        // probes per adjoint chunk (see msl_diff_backward)
   int prune_tiles = 0;  // band pruning (see the propagation passes)
   bool zb_rows = false;  // rows ny/4 .. 3ny/4 - 1 outside the propagator band
@@ -98,8 +98,8 @@ ColArgs cargs(msl_diff* d, const float2* src, float2* dst, float scale) {
 // at 1024 and writes the gradient rows itself, with no partial maps; on the
 // 1024^2 x 100-slice x 64-probe workload fwd+bwd 94.1 ms against 99.5 ms with
 // 4 probes per line and partial maps)
-void adj_groups(int nx, int nb, int& per_line, int& ngroups) {
-  const int lines = adj_lines(nx);
+void adj_groups(int nx, int nb, bool real_z, int& per_line, int& ngroups) {
+  const int lines = adj_lines(nx, real_z);
   per_line = std::max(1, std::min(16, (nb + lines - 1) / lines));
   ngroups = (nb + lines * per_line - 1) / (lines * per_line);
 }
@@ -116,16 +116,20 @@ int auto_chunk(int nx, int ny, int nprobe) {
   return nprobe;
 }
 
+// transmission exp(i sigma V_i) of slice i (1-based)
+float2* tslot(msl_diff* d, int i) { return d->tall.p + (size_t)(i - 1) * d->np(); }
+
 // slice i (1-based) forward: z_{i-1} (psi0 in real space for i = 1) -> z_i
 // at dst, through W (dst may be W: both passes work in place).  With wout
-// the row pass also stores w_i = IF_x(z_{i-1}) there.  The slice's
-// transmission exp(i sigma V_i) is evaluated once into tslice and read by the
-// row pass for every probe (evaluated per line, the same sincosf would run
-// once per probe).
-void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, float2* wout, hipStream_t s) {
-  HC(launch_transmission(d->V + (size_t)(i - 1) * d->np(), d->tslice.p, d->np(), d->sigma, s));
+// the row pass also stores w_i = IF_x(z_{i-1}) there.  With make_t the
+// slice's transmission exp(i sigma V_i) is evaluated once into its slot and
+// read by the row pass for every probe (evaluated per line, the same sincosf
+// would run once per probe); the backward's recomputation and adjoint passes
+// read the slots the forward wrote.
+void fwd_slice(msl_diff* d, int i, const float2* zprev, float2* dst, float2* wout, bool make_t, hipStream_t s) {
+  if (make_t) HC(launch_transmission(d->V + (size_t)(i - 1) * d->np(), tslot(d, i), d->np(), d->sigma, s));
   RowArgs r = rargs(d, zprev, d->W.p, i > 1, true);
-  r.t = d->tslice.p;
+  r.t = tslot(d, i);
   r.t_group = d->B;
   r.t_bstride = 0;
   if (i > 1) r.aux = wout;
@@ -213,7 +217,7 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     const size_t wave = (size_t)nprobe * nx * ny;
     d->W.alloc(wave);
     d->G.alloc(wave);
-    d->tslice.alloc((size_t)nx * ny);
+    d->tall.alloc((size_t)nslices * nx * ny);
     d->exitw.alloc(wave);
     if (d->S > 1) d->ckpt.alloc((size_t)(d->S - 1) * wave);
     d->seg.alloc((size_t)d->K * wave);
@@ -221,7 +225,11 @@ msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps,
     // finite from the start
     for (Buf<float2>* b : {&d->W, &d->G, &d->exitw, &d->ckpt, &d->seg})
       if (b->p) HC(hipMemset(b->p, 0, b->n * sizeof(float2)));
-    adj_groups(nx, nprobe, d->per_line, d->ngroups);
+    // partial maps for the larger group count of the two adjoint variants
+    int pl = 1, g0 = 1, g1 = 1;
+    adj_groups(nx, nprobe, true, pl, g0);
+    adj_groups(nx, nprobe, false, pl, g1);
+    d->ngroups = std::max(g0, g1);
     d->partial.alloc((size_t)d->ngroups * nx * ny);
     d->chunk = auto_chunk(nx, ny, nprobe);
   } catch (const DErr& e) {
@@ -255,7 +263,7 @@ msl_status msl_diff_info(const msl_diff* d, float* wl, float* sg, float* km, lon
   if (km) *km = d->kmax;
   if (ws) {
     const long long w = (long long)d->B * d->np() * 8;
-    *ws = w * (3 + (d->S - 1) + d->K) + (long long)d->ngroups * d->np() * 4;
+    *ws = w * (3 + (d->S - 1) + d->K) + (long long)d->ngroups * d->np() * 4 + (long long)d->ns * d->np() * 8;
   }
   return MSL_OK;
 }
@@ -291,7 +299,7 @@ msl_status diff_forward(msl_diff* d, const float* V, const void* psi0, float* in
     d->have_forward = false;
     for (int i = 1; i <= d->ns; ++i) {
       float2* dst = (keep && i % d->K == 0 && i < d->ns) ? ckpt_slot(d, i) : d->W.p;
-      fwd_slice(d, i, zprev, dst, keep && i > last_base ? wslot(d, i, last_base) : nullptr, s);
+      fwd_slice(d, i, zprev, dst, keep && i > last_base ? wslot(d, i, last_base) : nullptr, true, s);
       zprev = dst;
     }
     HC(launch_row(ROW_INV, RM_NONE, rargs(d, zprev, d->exitw.p, true, false), s));
@@ -330,7 +338,7 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
       if (!kept) {
         // recompute z_{base+1} .. z_{top-1} from the checkpoint, storing w_i
         for (int j = base + 1; j < top; ++j) {
-          fwd_slice(d, j, ztop, d->W.p, wslot(d, j, base), s);
+          fwd_slice(d, j, ztop, d->W.p, wslot(d, j, base), false, s);
           ztop = d->W.p;
         }
       }
@@ -340,8 +348,6 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
       for (int c0 = 0; c0 < d->B; c0 += d->chunk) {
         const int nb = std::min(d->chunk, d->B - c0);
         const long long off = (long long)c0 * np;
-        int per_line = 1, ngroups = 1;
-        adj_groups(d->nx, nb, per_line, ngroups);
         for (int i = top; i > base; --i) {
           ColArgs ca = cargs(d, d->G.p + off, d->G.p + off, d->nf * d->nf);
           ca.nbatch = nb;
@@ -356,7 +362,9 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
             a.Z = ztop + off;  // z_{top-1}: w_top is transformed inside the adjoint pass
             real_z = false;
           }
-          a.V = d->V + (size_t)(i - 1) * np;
+          int per_line = 1, ngroups = 1;
+          adj_groups(d->nx, nb, real_z, per_line, ngroups);
+          a.t = tslot(d, i);
           a.sigma = d->sigma;
           a.bstride = np;
           a.nx = d->nx; a.ny = d->ny; a.nbatch = nb;

@@ -242,7 +242,9 @@ typedef struct msl_diff msl_diff;
  * checkpoint_every K: the forward keeps every K-th wave and the backward
  * recomputes the others segment by segment; 0 keeps every slice's wave (no
  * recomputation; (3 + nslices) x nprobe x nx x ny x 8 bytes); < 0 picks the
- * longest segment whose waves fit in half the device's free memory. */
+ * longest segment whose waves fit in half the device's free memory.  The
+ * forward also keeps every slice's transmission exp(i sigma V_i) (nslices x
+ * nx x ny x 8 bytes), which the backward reads instead of V. */
 msl_status msl_diff_create(msl_diff** out, int device, int nx, int ny, float ps, float kV, float dz, int nslices,
                            int nprobe, int checkpoint_every);
 void msl_diff_destroy(msl_diff* d);

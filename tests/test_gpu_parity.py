@@ -452,7 +452,7 @@ def test_every_pass_variant_matches_oracle_in_batched_stem(fp):
     o.bin()
     want = o.stem_batch(px, py, dz, ns, 10.0, 200.0)["adf"]
     L = lib()
-    nrow, ncol = 3, 7
+    nrow, ncol = 4, 7
     try:
         for rv in range(nrow):
             for cv in range(ncol):

@@ -253,8 +253,9 @@ def test_diff_amorphous_carbon_1024_100_slices_matches_fp64_adjoint(fp):
     loss = ((I - target) ** 2).sum()
     loss.backward()
     torch.cuda.synchronize()
-    # 9 checkpoints, one segment of 10 waves and 3 working waves (+ one partial map per probe group)
-    assert eng.info()["workspace_bytes"] <= (3 + 9 + 10) * B * N * N * 8 + B * N * N * 4
+    # 9 checkpoints, one segment of 10 waves and 3 working waves, the transmission of
+    # every slice (+ one partial map per probe group)
+    assert eng.info()["workspace_bytes"] <= (3 + 9 + 10) * B * N * N * 8 + ns * N * N * 8 + B * N * N * 4
     Ig = I.detach().cpu().numpy()
     gI = I.grad.detach().cpu().numpy().astype(np.float64)
     g = Vt.grad.cpu().numpy()
