@@ -90,10 +90,10 @@ def diff_ratio(reps=3):
     min distance 1.4 A), 64 probes (8x8 raster, 25 mrad, 300 kV), loss
     sum (|psi|^2 - I_target)^2 with I_target from a seeded +-2 % perturbed
     potential.  Timed twice on the same inputs: with checkpoints every 10
-    slices (11 GiB, each segment recomputed once in the backward; the
-    headline ratio, the workload's own setting) and with every slice's wave
-    kept in HBM (checkpoint_every = 0: 51.5 GiB of workspace, no
-    recomputation)."""
+    slices (11.8 GiB with every slice's transmission, each segment
+    recomputed once in the backward; the headline ratio, the workload's own
+    setting) and with every slice's wave kept in HBM (checkpoint_every = 0:
+    52.3 GiB of workspace, no recomputation)."""
     import numpy as np
     import torch
     from cudamultislice_amd import MultisliceSimulation, POT_PROJECTED
@@ -357,8 +357,8 @@ def main():
     ap.add_argument("--profile-configs", type=int, default=2,
                     help="configurations run after the timed region with per-kernel HIP-event timing")
     ap.add_argument("--per-probe", type=int, default=8, help="probes timed on the per-probe path (0: skip)")
-    ap.add_argument("--cpu-probes", type=int, default=64)
-    ap.add_argument("--cpu-slices", type=int, default=40)
+    ap.add_argument("--cpu-probes", type=int, default=128)
+    ap.add_argument("--cpu-slices", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--no-tem", action="store_true", help="skip the plane-wave TEM cases and the rocFFT reference")

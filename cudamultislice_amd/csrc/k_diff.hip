@@ -63,46 +63,64 @@ __global__ void __launch_bounds__((AdjCfg<N, REAL_Z>::L * Plan<N>::T)) k_row_adj
   float acc[E];
 #pragma unroll
   for (int c = 0; c < E; ++c) acc[c] = 0.f;
+  // Software pipeline (PIPE): with two waves per SIMD little latency is
+  // hidden by other waves, so each probe's stored wave and the next probe's
+  // G row are requested before this probe's transforms (236 VGPRs at 1024).
+  // The variants that also transform z, or read unpruned rows, would need
+  // more than 256 VGPRs that way: they load G at the top of the iteration
+  // and w after the first transform.  Lines past the batch read wave 0 and
+  // are zeroed after the loads: unconditional loads issue together instead
+  // of one branch each.
+  constexpr bool PIPE = REAL_Z && PRUNE;
+  const float2 zero = make_float2(0.f, 0.f);
+  auto row_of = [&](int q) {
+    const int p = (g * a.per_line + q) * LINES + ln;
+    return (long long)(p < a.nbatch ? p : 0) * a.bstride + (long long)y * N;
+  };
+  auto load_g = [&](float2 (&v)[E], long long r) {
+#pragma unroll
+    for (int c = 0; c < E; ++c) v[c] = (!PRUNE || band_block<E>(c)) ? a.G[r + t + c * T] : zero;
+  };
+  // the stored real-space waves are read once: past the caches, so the
+  // gradient rows stay resident for the next column pass
+  auto load_w = [&](float2 (&v)[E], long long r) {
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      if constexpr (REAL_Z) v[c] = nt_load(&a.Z[r + t + c * T]);
+      else v[c] = (!PRUNE || band_block<E>(c)) ? a.Z[r + t + c * T] : zero;
+    }
+  };
+  float2 gn[E];
+  if constexpr (PIPE) load_g(gn, row_of(0));
+  // band-limited inputs (PRUNE): the first butterflies skip the zero blocks
+  constexpr bool ZIN = PRUNE && E % 4 == 0 && Plan<N>::R1 % 4 == 0;
   for (int q = 0; q < a.per_line; ++q) {
     const int p = (g * a.per_line + q) * LINES + ln;
     const bool ok = p < a.nbatch;
-    float2 gu[E];
-    // lines past the batch read wave 0 and are zeroed after the loads:
-    // unconditional loads issue together instead of one branch each
-    const long long ro = (long long)(ok ? p : 0) * a.bstride + (long long)y * N;
-    const float2 zero = make_float2(0.f, 0.f);
+    const long long ro = row_of(q);
+    float2 gu[E], w[E];
+    if constexpr (PIPE) {
 #pragma unroll
-    for (int c = 0; c < E; ++c) {
-      const bool band = !PRUNE || band_block<E>(c);
-      gu[c] = band ? a.G[ro + t + c * T] : zero;
+      for (int c = 0; c < E; ++c) gu[c] = gn[c];
+      load_w(w, ro);
+      if (q + 1 < a.per_line) load_g(gn, row_of(q + 1));
+    } else {
+      load_g(gu, ro);
     }
 #pragma unroll
     for (int c = 0; c < E; ++c) gu[c] = ok ? gu[c] : zero;
-    // band-limited inputs (PRUNE): the first butterflies skip the zero blocks
-    constexpr bool ZIN = PRUNE && E % 4 == 0 && Plan<N>::R1 % 4 == 0;
     fft_line<Plan<N>, N, +1, S, LL, ZIN>(gu, t, stw, L);  // G_u = IF_x(G_a)
-    {
-      // w_i is read only now (not held through the transform above); the
-      // stored real-space waves are read once: past the caches, so the
-      // gradient rows stay resident for the next column pass
-      float2 w[E];
+    if constexpr (!PIPE) load_w(w, ro);
 #pragma unroll
-      for (int c = 0; c < E; ++c) {
-        const bool band = !PRUNE || band_block<E>(c);
-        if constexpr (REAL_Z) w[c] = nt_load(&a.Z[ro + t + c * T]);
-        else w[c] = band ? a.Z[ro + t + c * T] : zero;
-      }
+    for (int c = 0; c < E; ++c) w[c] = ok ? w[c] : zero;
+    if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S, LL, ZIN>(w, t, stw, L);  // w = IF_x(z_{i-1})
 #pragma unroll
-      for (int c = 0; c < E; ++c) w[c] = ok ? w[c] : zero;
-      if constexpr (!REAL_Z) fft_line<Plan<N>, N, +1, S, LL, ZIN>(w, t, stw, L);  // w = IF_x(z_{i-1})
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        const float2 gw = cmulc(gu[c], tr[c]);  // conj(t) G_u
-        acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
-        gu[c] = gw;
-      }
+    for (int c = 0; c < E; ++c) {
+      const float2 gw = cmulc(gu[c], tr[c]);  // conj(t) G_u
+      acc[c] += -a.sigma * (gw.x * w[c].y - gw.y * w[c].x);
+      gu[c] = gw;
     }
-    fft_line<Plan<N>, N, -1, S>(gu, t, stw, L);              // G_{z_{i-1}} = F_x(G_w)
+    fft_line<Plan<N>, N, -1, S>(gu, t, stw, L);  // G_{z_{i-1}} = F_x(G_w)
     if (ok) {
 #pragma unroll
       for (int c = 0; c < E; ++c)

@@ -62,12 +62,13 @@ static int lg2(int n) {
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 1-4 GiB
-// of waves): rows of 512 take the alternative plan (1.08x over 1); at 1024 the
-// alternative plan holds 256 VGPRs (one wave per SIMD) when the row pass
-// evaluates exp(i sigma V) itself, so 1024 keeps 1.  Columns: 512 and 1024
-// half-width tiles (1.06x, 1.03x over 0), 256 and 2048 variant 0.
-static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 1, 1};
-static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 3, 0};
+// of waves): rows of 512 take the alternative plan (0.67 vs 0.71 ms per 1024
+// waves for 1), rows of 1024 the 512-thread workgroups (0.81 vs 0.91 ms per
+// 256 waves).  Columns: 512 half-width tiles (1.06x over 0); 256, 1024 and
+// 2048 variant 0 (1024: 0.54 vs 0.62 ms per 256 waves for the half-width
+// tiles once the inverse skipped the propagator's zero rows).
+static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 3, 1};
+static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 0, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
 void set_pass_variant(int kind, int n, int v) {
@@ -243,6 +244,14 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
         }
         asm volatile("" ::: "memory");
       }
+    }
+  } else if constexpr (MUL == RM_SEED) {
+    // G = 2 dL/dI psi of an intensity loss (dL/dRe + i dL/dIm), in the row's
+    // own position of the seed array
+    if (ok) {
+      const float* sr = a.seed + roff(a.src_bstride);
+#pragma unroll
+      for (int c = 0; c < E; ++c) v[c] = cscale(v[c], 2.f * sr[t + c * T] * a.scale);
     }
   } else if constexpr (MUL == RM_T) {
     if (ok) {
@@ -894,11 +903,14 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0) | (a.aux ? STORE_AUX : 0);
   if (a.aux && mode != ROW_INV_MUL_FWD) return hipErrorInvalidValue;
   if (mode == ROW_INV_FD_FWD && (!a.fd_pm || !a.fd_p0 || !a.t)) return hipErrorInvalidValue;
+  if (mul == RM_SEED && (mode != ROW_FWD || !a.seed)) return hipErrorInvalidValue;
 #define MSL_ROW(M, U, F) hipLaunchKernelGGL((k_row<N, M, U, F, RV>), grid, block, 0, s, a)
   if (mode == ROW_FWD && mul == RM_NONE && fl == 0) MSL_ROW(ROW_FWD, RM_NONE, 0);
   else if (mode == ROW_FWD && mul == RM_NONE && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_NONE, PRUNE_OUT);
   else if (mode == ROW_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_FWD, RM_T, 0);
   else if (mode == ROW_FWD && mul == RM_T && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_T, PRUNE_OUT);
+  else if (mode == ROW_FWD && mul == RM_SEED && fl == 0) MSL_ROW(ROW_FWD, RM_SEED, 0);
+  else if (mode == ROW_FWD && mul == RM_SEED && fl == PRUNE_OUT) MSL_ROW(ROW_FWD, RM_SEED, PRUNE_OUT);
   else if (mode == ROW_INV && mul == RM_NONE && fl == 0) MSL_ROW(ROW_INV, RM_NONE, 0);
   else if (mode == ROW_INV && mul == RM_NONE && fl == PRUNE_IN) MSL_ROW(ROW_INV, RM_NONE, PRUNE_IN);
   else if (mode == ROW_INV_MUL_FWD && mul == RM_T && fl == 0) MSL_ROW(ROW_INV_MUL_FWD, RM_T, 0);

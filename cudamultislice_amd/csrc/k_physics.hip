@@ -876,16 +876,4 @@ hipError_t launch_transmission(const float* V, float2* t, long long n, float sig
   return hipGetLastError();
 }
 
-// ============================================================== adjoint seed
-// G = dL/dRe + i dL/dIm of the exit wave for a loss of its intensity
-__global__ void k_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  g[i] = cscale(psi[i], 2.f * dLdI[i]);
-}
-hipError_t launch_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n, int nbatch, hipStream_t s) {
-  hipLaunchKernelGGL(k_seed_adjoint, dim3(nblk(n * nbatch, 256)), dim3(256), 0, s, psi, dLdI, g, n * nbatch);
-  return hipGetLastError();
-}
-
 }  // namespace msl

@@ -326,9 +326,11 @@ msl_status msl_diff_backward(msl_diff* d, const float* dLdI, float* gradV, void*
     // default-stream work of the process (as torch's default stream)
     const hipStream_t s = (hipStream_t)stream;
     const long long np = d->np();
-    // G at the exit: 2 dL/dI psi_n, then the adjoint of the final IF_x
-    HC(launch_seed_adjoint(d->exitw.p, dLdI, d->G.p, np, d->B, s));
-    HC(launch_row(ROW_FWD, RM_NONE, rargs(d, d->G.p, d->G.p, false, true), s));
+    // G at the exit, 2 dL/dI psi_n, through the adjoint of the final IF_x
+    // (one row pass forms and transforms it)
+    RowArgs sa = rargs(d, d->exitw.p, d->G.p, false, true);
+    sa.seed = dLdI;
+    HC(launch_row(ROW_FWD, RM_SEED, sa, s));
     for (int sg = d->S - 1; sg >= 0; --sg) {
       const int base = sg * d->K;
       const int top = std::min(base + d->K, d->ns);

@@ -15,8 +15,10 @@ enum RowMode { ROW_FWD = 0, ROW_INV = 1, ROW_INV_MUL_FWD = 2, ROW_INV_FD_FWD = 3
 // (src), psi+ from it, psi- (fd_pm), psi (fd_p0) and V (t, real part, slot
 // indexing as RM_T), then FFT_x of psi+ into dst.
 // pointwise factor applied between the (optional) inverse and forward
-// transforms: none, or a stored transmission t
-enum RowMul { RM_NONE = 0, RM_T = 1 };
+// transforms: none, a stored transmission t, or (RM_SEED, with ROW_FWD) twice
+// a real array in the wave's own layout: the gradient 2 dL/dI psi of an
+// intensity loss, transformed in the same pass
+enum RowMul { RM_NONE = 0, RM_T = 1, RM_SEED = 2 };
 
 // The launch structures are written once as field lists parameterised by a
 // pointer qualifier G: empty for the host-visible structs below, the global
@@ -42,7 +44,8 @@ enum RowMul { RM_NONE = 0, RM_T = 1 };
   /* ROW_INV_FD_FWD: psi- and psi in real space (dst indexing), slice step and wavelength */             \
   const float2 G* fd_pm;                                                                                 \
   const float2 G* fd_p0;                                                                                 \
-  float fd_dz, fd_wavel;
+  float fd_dz, fd_wavel;                                                                                 \
+  const float G* seed;       /* RM_SEED: real factor per element, src indexing */
 struct RowArgs {
   MSL_ROW_ARGS()
 };
@@ -276,9 +279,5 @@ hipError_t launch_sum_maps(const float* in, int nbatch, long long n, float* out,
 hipError_t launch_fd_update(const float2* grad, const float2* psim, float2* psip, const float2* psi,
                             const float2* V, long long n, long long vper, float sigma, float wavel, float dz,
                             hipStream_t s);
-
-// differentiable multislice helpers
-hipError_t launch_seed_adjoint(const float2* psi, const float* dLdI, float2* g, long long n, int nbatch,
-                               hipStream_t s);
 
 }  // namespace msl
