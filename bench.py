@@ -412,10 +412,10 @@ def main():
     log(f"[bench] {atoms[0].size} atoms, {nslices} slices, {nprobe} probes, 512^2")
 
     # batch width: the autotuner's choice (one chunk per candidate width over
-    # 8 slices, before the timed region) unless --batch fixes it
+    # 16 slices, before the timed region) unless --batch fixes it
     tuned = None
     if args.batch == 0 and not args.no_tune_batch:
-        best, cost = sim.autotune_batch(px, py, dz, nslices=8, adf=(60.0, 200.0))
+        best, cost = sim.autotune_batch(px, py, dz, nslices=16, adf=(60.0, 200.0))
         tuned = {"best": best, "us_per_probe_slice": {str(k): round(v, 4) for k, v in cost.items()}}
         log(f"[bench] batch width autotuner: {tuned}")
     # warm-up configurations use seeds distinct from the timed ones (and

@@ -59,31 +59,34 @@ static int lg2(int n) {
 //           one is transformed, so the strided column loads overlap compute
 //         6 the tile staged through LDS, then one column per wave with
 //           wave-local exchanges (two workgroup barriers per tile)
+//         7 as 2 with half as many columns per workgroup
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 1-4 GiB
 // of waves): rows of 512 take the alternative plan (0.67 vs 0.71 ms per 1024
 // waves for 1), rows of 1024 the 512-thread workgroups (0.81 vs 0.91 ms per
-// 256 waves).  Columns: 512 half-width tiles (1.06x over 0); 256, 1024 and
+// 256 waves).  Columns: 512 half-width tiles on the alternative plan (0.48
+// vs 0.51 ms per 1024 waves for half-width tiles on the default plan, 1 %
+// more probe-slices per second on the 512^2 STEM workload); 256, 1024 and
 // 2048 variant 0 (1024: 0.54 vs 0.62 ms per 256 waves for the half-width
 // tiles once the inverse skipped the propagator's zero rows).
 static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 3, 1};
-static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 3, 0, 0};
+static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 7, 0, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
 void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 4 : 7; }
+int num_pass_variants(int kind) { return kind == 0 ? 4 : 8; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
 template <int N, int CV> struct ColCfg {
-  using P = typename std::conditional<CV == 2, AltPlan<N>, Plan<N>>::type;
+  using P = typename std::conditional<CV == 2 || CV == 7, AltPlan<N>, Plan<N>>::type;
   static constexpr bool INTERLEAVED = CV != 1;
   static constexpr int T = P::T;
-  static constexpr int C = CV == 3 ? ColTile0<N>::C / 2 : ColTile0<N>::C;
+  static constexpr int C = (CV == 3 || CV == 7) ? ColTile0<N>::C / 2 : ColTile0<N>::C;
   static constexpr int SH = INTERLEAVED ? 0 : 4;
   // [col][row] line stride in float2: span + padding so consecutive columns
   // start two banks apart (stride = 2 mod 32 float2)
@@ -1037,6 +1040,7 @@ static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int 
     if constexpr (N >= 128 && N <= 1024) return col_wl<N>(mode, mul, a, s);
     else return col_nv<N, 0>(mode, mul, a, s);
   }
+  if (var == 7) return col_nv<N, 7>(mode, mul, a, s);
   if (var == 5) return col_stream<N>(mode, mul, a, s);
   if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
   if (var == 3) return col_nv<N, 3>(mode, mul, a, s);

@@ -173,13 +173,22 @@ void kirkland_transform(const float* fp, float* zt) {
   }
 }
 
+// The innermost function of the potential kernels: one square root, six
+// base-2 exponentials and a reciprocal per sample, taken as the hardware
+// instructions (about 1 ulp each).  The library forms of sqrtf, exp2f and the
+// division are correctly rounded over the whole range, denormals included,
+// at 82 instructions per sample against about 30 here; their difference is
+// ~1e-7 of the potential.  Exponentials below 2^-126 flush to zero: terms of
+// that size are below the FP32 resolution of the O(1)..O(1e3) V sums.
 __device__ __forceinline__ float vatom_dev(const float* c, float r2) {
-  float r = sqrtf(r2);
+  float r = __builtin_amdgcn_sqrtf(r2);
   r = fmaxf(r, 0.25f);
   const float rr = r * r;
-  const float sl = c[0] * exp2f(-c[1] * r) + c[2] * exp2f(-c[3] * r) + c[4] * exp2f(-c[5] * r);
-  const float sg = c[6] * exp2f(-c[7] * rr) + c[8] * exp2f(-c[9] * rr) + c[10] * exp2f(-c[11] * rr);
-  return sl / r + sg;
+  const float sl = c[0] * __builtin_amdgcn_exp2f(-c[1] * r) + c[2] * __builtin_amdgcn_exp2f(-c[3] * r) +
+                   c[4] * __builtin_amdgcn_exp2f(-c[5] * r);
+  const float sg = c[6] * __builtin_amdgcn_exp2f(-c[7] * rr) + c[8] * __builtin_amdgcn_exp2f(-c[9] * rr) +
+                   c[10] * __builtin_amdgcn_exp2f(-c[11] * rr);
+  return sl * __builtin_amdgcn_rcpf(r) + sg;
 }
 
 // Abramowitz & Stegun 9.8.1-9.8.6 in FP32 (the forms of U.cu:41-111)
@@ -284,6 +293,7 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
   __shared__ float sx[ACAP], sy[ACAP], sz[ACAP];
   __shared__ int sZ[ACAP];
   __shared__ int rs[64], rl[65];
+  __shared__ float zsm[21];  // the slice's z samples (modes 0 and 3), index m
   const int tid = threadIdx.x;
   for (int i = tid; i < 103 * 24; i += 256) zt[i] = a.ztab[i];
   const int ix = tx * TX + (tid % TX), iy = ty * TY + (tid / TX);
@@ -321,6 +331,10 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
   const int i0 = max(0, (int)floorf((tx0 - reach) / wx) - 1), i1 = min(49, (int)floorf((tx1 + reach) / wx) + 1);
   const int j0 = max(0, (int)floorf((ty0 - reach) / wy) - 1), j1 = min(49, (int)floorf((ty1 + reach) / wy) + 1);
   const float h20 = a.dz / 20.0f;
+  // the sample positions depend on the slice only: computed once per
+  // workgroup (each an FP32 division, as the reference takes it) instead of
+  // once per atom and pixel
+  if (tid >= 1 && tid <= 20) zsm[tid] = a.mode == 3 ? (tid <= 10 ? sample_z_rs(zf, tid) : 0.f) : sample_z(zf, tid, a.dz);
   float sum = 0.f;
   __syncthreads();
   for (int k = k0; k <= k1; ++k) {
@@ -374,21 +388,29 @@ __device__ __forceinline__ void potential_tile(const A& a, int tx, int ty, int s
             // samples m with |zf - m dz / 20 - zr| < sqrt(5 - rxy), widened by
             // one sample on each side; inside that range each sample takes
             // the exact FP32 cutoff test, so the set summed is the 20-sample
-            // loop's (the samples outside fail it by more than rounding)
-            const float hz = sqrtf(5.0f - rxy), u = 20.0f / a.dz;
+            // loop's (the samples outside fail it by more than rounding, and
+            // by far more than the hardware square root's last bit)
+            const float hz = __builtin_amdgcn_sqrtf(5.0f - rxy), u = 20.0f / a.dz;
             const int mlo = max(1, (int)floorf((zf - zr - hz) * u) - 1);
             const int mhi = min(20, (int)ceilf((zf - zr + hz) * u) + 1);
+            // the atom's coefficients in registers for all its samples
+            float cr[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) cr[i] = c[i];
             for (int m = mlo; m <= mhi; ++m) {
-              const float r2 = dist2_z(rxy, sample_z(zf, m, a.dz), zr);
-              if (r2 < 5.0f) sum += h20 * vatom_dev(c, r2);
+              const float r2 = dist2_z(rxy, zsm[m], zr);
+              if (r2 < 5.0f) sum += h20 * vatom_dev(cr, r2);
             }
           } else if (a.mode == 1) {
             sum += vzatom_dev(c, rxy);
           } else if (a.mode == 3) {  // real-space slab of 1 A, 10 samples
             const float zr = sz[l];
+            float cr[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) cr[i] = c[i];
             for (int m = 1; m <= 10; ++m) {
-              const float r2 = dist2_z(rxy, sample_z_rs(zf, m), zr);
-              if (r2 < 5.0f) sum += 0.1f * vatom_dev(c, r2);
+              const float r2 = dist2_z(rxy, zsm[m], zr);
+              if (r2 < 5.0f) sum += 0.1f * vatom_dev(cr, r2);
             }
           } else {
             const float r2 = dist2_z(rxy, zf, sz[l]);

@@ -273,9 +273,9 @@ class MultisliceSimulation:
 
     def autotune_batch(self, posx, posy, dz, nslices=8, adf=(60.0, 200.0)):
         """Times one chunk of B of the given probes over nslices slices for
-        B = 64, 128, ... and keeps the fastest width (per probe-slice) for later
-        stem_batched(batch=0) runs on this grid; returns (best, {B: us per
-        probe-slice})."""
+        B = 64, 128, ... and keeps, for later stem_batched(batch=0) runs on
+        this grid, the widest B within 2 % of the fastest per probe-slice;
+        returns (best, {B: us per probe-slice})."""
         posx = np.ascontiguousarray(posx, np.int32)
         posy = np.ascontiguousarray(posy, np.int32)
         adf_out = np.zeros(posx.size, np.float32)

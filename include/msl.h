@@ -219,8 +219,9 @@ typedef struct {
 msl_status msl_stem_batched(msl_sim* sim, const msl_stem_batch_desc* desc);
 /* batch width autotuner: runs one chunk of B of the descriptor's probes over
  * nslices slices for B = 64, 128, ... up to nprobe (and ~32 GiB of waves),
- * and keeps the fastest per probe-slice as the width of later batch = 0 runs
- * on this grid size.  On input *ncand is the capacity of widths /
+ * the faster of two timed runs each, and keeps the widest B within 2 % of the
+ * fastest per probe-slice as the width of later batch = 0 runs on this grid
+ * size.  On input *ncand is the capacity of widths /
  * us_per_probe_slice (may be 0); on output the number of candidates. */
 msl_status msl_autotune_batch(msl_sim* sim, const msl_stem_batch_desc* desc, int nslices, int* best, int* widths,
                               float* us_per_probe_slice, int* ncand);
