@@ -357,8 +357,8 @@ msl_status msl_fft2_device(int nx, int ny, int nbatch, void* data, int inverse, 
  * rewrites that array: calls that overlap on one GPU race and must be
  * serialised by the caller. */
 msl_status msl_fft2_device_noalloc(int nx, int ny, int nbatch, void* data, int inverse, void* stream);
-/* pixel tile of the potential kernel: 0 16x16 (default), 1 32x8, 2 64x4,
- * 3 8x32 threads; all give the same bits.  msl_autotune_potential times
+/* pixel tile of the potential kernel: 0 16x16, 1 32x8, 2 64x4,
+ * 3 8x32 threads (default); all give the same bits.  msl_autotune_potential times
  * each on the handle's binned atoms and grid (nslices slices of thickness
  * dz from the top) and keeps the fastest whose potentials equal the 16x16
  * tile's; ms_per_variant (optional, 4 floats) gets the times (-1 where
