@@ -47,6 +47,8 @@ static int lg2(int n) {
 //         2 as 1 with the alternative plan (AltPlan: other radices and
 //           threads per line)
 //         3 as 1 with 512-thread workgroups (twice the lines)
+//         4 a workgroup per row looping over waves, transmission in registers
+//           (k_row_pl; IFFT -> shared t -> FFT only, variant 2 elsewhere)
 // column: 0 [row][col] interleaved LDS, workgroup barriers
 //         1 [col][row] LDS with pad 1/16, workgroup barriers
 //         2 as 0 with the alternative plan
@@ -54,9 +56,9 @@ static int lg2(int n) {
 //           per CU to overlap the barriers)
 //         4 as 0 with two LDS copies used by alternate exchanges (one
 //           barrier per exchange instead of two)
-//         5 as 0 in persistent workgroups that stream the next tile into a
-//           second LDS buffer (direct global-to-LDS loads) while the current
-//           one is transformed, so the strided column loads overlap compute
+//         5 a workgroup per tile looping over waves, propagator factors in
+//           registers (k_col_pl; the propagation pass only, the length's
+//           default elsewhere)
 //         6 the tile staged through LDS, then one column per wave with
 //           wave-local exchanges (two workgroup barriers per tile)
 //         7 as 2 with half as many columns per workgroup
@@ -65,12 +67,15 @@ static int lg2(int n) {
 // defaults from msl_autotune_passes on MI355X (band-pruned passes, 1-4 GiB
 // of waves): rows of 512 take the alternative plan (0.67 vs 0.71 ms per 1024
 // waves for 1), rows of 1024 the 512-thread workgroups (0.81 vs 0.91 ms per
-// 256 waves).  Columns: 512 half-width tiles on the alternative plan (0.48
+// 256 waves); rows of 512 and 1024 then went to variant 4 (one workgroup per
+// row looping over waves; 512: 0.58 vs 0.69 ms per 1024 waves for 2, 1024:
+// 0.78 vs 0.85 ms per 256 waves for 3; 1.131 vs 1.080e6 probe-slice/s on the
+// 512^2 STEM workload).  Columns: 512 half-width tiles on the alternative plan (0.48
 // vs 0.51 ms per 1024 waves for half-width tiles on the default plan, 1 %
 // more probe-slices per second on the 512^2 STEM workload); 256, 1024 and
 // 2048 variant 0 (1024: 0.54 vs 0.62 ms per 256 waves for the half-width
 // tiles once the inverse skipped the propagator's zero rows).
-static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 2, 3, 1};
+static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 4, 4, 1};
 static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 7, 0, 0};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
@@ -78,7 +83,7 @@ void set_pass_variant(int kind, int n, int v) {
   if (kind == 0) g_row_var[lg2(n)] = v;
   else g_col_var[lg2(n)] = v;
 }
-int num_pass_variants(int kind) { return kind == 0 ? 4 : 8; }
+int num_pass_variants(int kind) { return kind == 0 ? 5 : 8; }
 
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
@@ -314,6 +319,79 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
     if (N % RowCfg<N, RV>::LINES == 0 && a.ny == N)
       blk = row_order(blockIdx.x, gridDim.x, a.nbatch, N / RowCfg<N, RV>::LINES);
   row_block<N, MODE, MUL, FL, RV>(a, blk, lds, stw);
+}
+
+// Variant 4, for IFFT_x -> t -> FFT_x with one transmission shared by every
+// wave of the launch (a batched STEM chunk, the differentiable forward): a
+// workgroup owns row y of LINES x per_line waves.  Each thread keeps its E
+// transmission values in registers for all of them (the other layouts load
+// them once per wave), and requests the next wave's row before transforming
+// the current one.
+template <int N> struct RowPL {
+  using P = typename std::conditional<(N <= 512), AltPlan<N>, Plan<N>>::type;
+  static constexpr int T = P::T;
+  static constexpr bool WAVE = T <= 64;
+  static constexpr int SH = 4;
+  static constexpr int LINES = (256 / T) > 0 ? 256 / T : 1;
+  static constexpr int LS = LdsLine<SH>::span(N) + 1;
+};
+
+template <int N, int FL>
+__global__ void __launch_bounds__((RowPL<N>::LINES * RowPL<N>::T)) k_row_pl(RowArgs a, int per_line) {
+  using Cfg = RowPL<N>;
+  using P = typename Cfg::P;
+  using S = typename std::conditional<Cfg::WAVE, WaveSync, BlockSync>::type;
+  using LL = LdsLine<Cfg::SH>;
+  constexpr int T = Cfg::T, E = N / T, LINES = Cfg::LINES, LS = Cfg::LS, NT = LINES * T;
+  __shared__ float2 lds[LINES * LS];
+  __shared__ float2 stw[N];
+  const int tid = threadIdx.x, t = tid % T, ln = tid / T;
+  for (int i = tid; i < N; i += NT) stw[i] = a.tw[i];
+  const int y = blockIdx.x, g = blockIdx.y;
+  float2 tr[E];
+  {
+    const float2* trow = a.t + (long long)y * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) tr[c] = trow[t + c * T];
+  }
+  __syncthreads();
+  const LL L{lds + ln * LS, 1};
+  const float2 zero = make_float2(0.f, 0.f);
+  // lines past the batch read wave 0 and store nothing
+  auto wave_of = [&](int q) { return (g * per_line + q) * LINES + ln; };
+  auto load = [&](float2 (&v)[E], int b) {
+    const float2* row = a.src + (long long)(b < a.nbatch ? b : 0) * a.src_bstride + (long long)y * N;
+#pragma unroll
+    for (int c = 0; c < E; ++c) v[c] = (!(FL & PRUNE_IN) || BandBlocks<E>::active(c)) ? row[t + c * T] : zero;
+  };
+  constexpr bool ZIN = (FL & PRUNE_IN) != 0 && E % 4 == 0 && P::R1 % 4 == 0;
+  float2 vn[E];
+  load(vn, wave_of(0));
+  for (int q = 0; q < per_line; ++q) {
+    const int b = wave_of(q);
+    float2 v[E];
+#pragma unroll
+    for (int c = 0; c < E; ++c) v[c] = vn[c];
+    if (q + 1 < per_line) load(vn, wave_of(q + 1));
+    fft_line<P, N, +1, S, LL, ZIN>(v, t, stw, L);
+    if constexpr ((FL & STORE_AUX) != 0) {
+      // the real-space wave for the adjoint pass, streamed past the caches
+      if (b < a.nbatch) {
+        float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
+#pragma unroll
+        for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[c]), a.scale);
+    fft_line<P, N, -1, S>(v, t, stw, L);
+    if (b < a.nbatch) {
+      float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
+#pragma unroll
+      for (int c = 0; c < E; ++c)
+        if (!(FL & PRUNE_OUT) || BandBlocks<E>::active(c)) row[t + c * T] = v[c];
+    }
+  }
 }
 
 // ------------------------------------------------------------------ columns
@@ -595,111 +673,67 @@ static hipError_t col_wl(int mode, int mul, const ColArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ streamed columns
-// Variant 5: a persistent workgroup walks the (wave, tile) items of the
-// launch; the [row][col] tile of the next item is fetched straight into the
-// other of two LDS buffers (global_load_lds, 16 bytes per lane: lane l of an
-// instruction lands at base + 16 l, i.e. C / 2 lanes per 16-column row) while
-// the current tile is transformed in registers, the current buffer serving as
-// the exchange space.  Arithmetic is that of variant 0.
-template <int N> struct ColStream {
-  // two tiles of C columns must fit the LDS, a workgroup at most 512 threads
-  // (room for the second buffer's addresses) and at least one wavefront
-  static constexpr int T = Plan<N>::T;
-  static constexpr int C0 = (N >= 1024) ? ColTile0<N>::C : ColTile0<N>::C / 2;
-  static constexpr int C = C0 * T >= 64 ? C0 : 64 / T;
-  static constexpr int THREADS = C * T, E = N / T;
-  static constexpr int LANES_PER_ROW = C / 2;           // 16-byte lanes per tile row
-  static constexpr int ROWS_PER_OP = 64 / LANES_PER_ROW;  // rows one wave instruction covers
-  static constexpr int OPS = N / ROWS_PER_OP;             // instructions per tile
-  static constexpr int WAVES = THREADS / 64;
-  static constexpr int OPS_PER_WAVE = OPS / WAVES;        // = E / 2
-  static_assert(C % 2 == 0 && 64 % LANES_PER_ROW == 0 && OPS % WAVES == 0, "tile does not map onto waves");
-};
+// ------------------------------------------------------------------ columns looping over waves
+// Variant 5, for the propagation pass (FFT_y -> P -> IFFT_y): a workgroup
+// owns one column tile and loops over per_tile waves of the batch; each
+// thread forms its propagator factors once (the other layouts load and form
+// them again for every wave).  Tile, plan and LDS layout of the length's
+// default variant (7 at 512, 0 otherwise), held to four waves per SIMD (112
+// VGPRs at 512).  Requesting the next wave's tile during the transforms as
+// well took 174 VGPRs, two waves per SIMD, and ran slower than variant 7.
+template <int N> struct ColPL { static constexpr int CV = N == 512 ? 7 : 0; };
 
-typedef __attribute__((address_space(3))) void* lds_vptr;
-
-// s_waitcnt vmcnt(n) leaving the other counters free (GFX9 encoding:
-// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
-template <int n> __device__ __forceinline__ void wait_vmcnt() {
-  static_assert(n >= 0 && n < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
-// Workgroup barrier for LDS traffic only: waits for this wave's LDS
-// operations, not for its outstanding global loads (the streamed next tile),
-// which a fenced __syncthreads() would drain.
-struct LdsBarrier {
-  __device__ static __forceinline__ void sync() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-};
-
-template <int N>
-__device__ __forceinline__ void stream_tile(const ColArgs& a, int item, int ntl, float2* buf) {
-  using S = ColStream<N>;
-  const int b = item / ntl, ti = item - b * ntl;
-  const int tile = (a.tiles_lo > 0 && ti >= a.tiles_lo) ? ti + a.nx / S::C - 2 * a.tiles_lo : ti;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rr = lane / S::LANES_PER_ROW, cc = 2 * (lane % S::LANES_PER_ROW);
-  const float2* src = a.src + (long long)b * a.src_bstride + tile * S::C + cc;
-#pragma unroll
-  for (int k = 0; k < S::OPS_PER_WAVE; ++k) {
-    const int r0 = (wave * S::OPS_PER_WAVE + k) * S::ROWS_PER_OP;
-    __builtin_amdgcn_global_load_lds((const void*)(src + (long long)(r0 + rr) * a.nx),
-                                     (lds_vptr)(buf + r0 * S::C), 16, 0, 0);
-  }
-}
-
-// one item: wait for its tile in cur, transform it, store it; the tile of
-// the following item streams into nxt meanwhile.  cur and nxt are distinct
-// (restrict), which lets the compiler keep the transform's LDS traffic from
-// waiting on the stream into nxt.
-template <int N, int MODE, int MUL>
-__device__ __forceinline__ void stream_item(const ColArgs& a, int it, int next, int items, int ntl,
-                                            float2* __restrict__ cur, float2* __restrict__ nxt, const float2* tw) {
-  using S = ColStream<N>;
-  constexpr int T = S::T, E = S::E, C = S::C;
-  const int col = threadIdx.x % C, t = threadIdx.x / C;
-  if (next < items) {
-    stream_tile<N>(a, next, ntl, nxt);
-    wait_vmcnt<S::OPS_PER_WAVE>();  // this wave's loads of the current tile are done
-  } else {
-    wait_vmcnt<0>();
-  }
-  LdsBarrier::sync();  // every wave's share of the current tile has landed
-  float2 v[E];
-#pragma unroll
-  for (int c = 0; c < E; ++c) v[c] = cur[(t + c * T) * C + col];
-  LdsBarrier::sync();  // the exchanges below reuse the current buffer
-  const int b = it / ntl, ti = it - b * ntl;
-  const int tile = (a.tiles_lo > 0 && ti >= a.tiles_lo) ? ti + a.nx / C - 2 * a.tiles_lo : ti;
-  const int x = tile * C + col;
-  col_body<Plan<N>, N, MODE, MUL, LdsBarrier>(v, t, x, a, LdsLine<0>{cur + col, C}, tw);
-  float2* dst = a.dst + (long long)b * a.dst_bstride + x;
-  const long long nx = a.nx;
-#pragma unroll
-  for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
-  LdsBarrier::sync();  // the current buffer is refilled two items later
-}
-
-template <int N, int MODE, int MUL>
-__global__ void __launch_bounds__((ColStream<N>::THREADS)) k_col_stream(ColArgs a, int ntl, int items) {
-  __shared__ float2 buf0[N * ColStream<N>::C];
-  __shared__ float2 buf1[N * ColStream<N>::C];
+template <int N, int MUL>
+__global__ void __launch_bounds__((ColCfg<N, ColPL<N>::CV>::THREADS), 4) k_col_pl(ColArgs a, int per_tile) {
+  using Cfg = ColCfg<N, ColPL<N>::CV>;
+  using P = typename Cfg::P;
+  using LL = LdsLine<Cfg::SH>;
+  constexpr int T = Cfg::T, E = N / T, C = Cfg::C;
+  __shared__ float2 lds[Cfg::LDS];
   __shared__ float2 stw[N];
-  // trip k takes item k G + r: within a trip each XCD holds a contiguous
-  // range of items (neighbouring tiles share an L2)
-  const int G = gridDim.x;
-  int item = (int)xcd_remap(blockIdx.x, G);
-  if (item >= items) return;
-  for (int i = threadIdx.x; i < N; i += ColStream<N>::THREADS) stw[i] = a.tw[i];
+  for (int i = threadIdx.x; i < N; i += Cfg::THREADS) stw[i] = a.tw[i];
+  const long long gid = xcd_remap((long long)blockIdx.y * gridDim.x + blockIdx.x, (long long)gridDim.x * gridDim.y);
+  const int bx = (int)(gid % gridDim.x), g = (int)(gid / gridDim.x);
+  const int col = threadIdx.x % C, t = threadIdx.x / C;
+  const int tile = (a.tiles_lo > 0 && bx >= a.tiles_lo) ? bx + a.nx / C - 2 * a.tiles_lo : bx;
+  const int x = tile * C + col;
+  const long long nx = a.nx;
+  constexpr bool CONJ = MUL == CM_PROPCONJ || MUL == CM_PROPCONJ_ZB;
+  // the propagator rows N/4 .. 3N/4 - 1 are zero (canonical blocks E/4 .. 3E/4 - 1)
+  constexpr bool ZB = (MUL == CM_PROP_ZB || MUL == CM_PROPCONJ_ZB) && E % 4 == 0 && P::R1 % 4 == 0;
+  // P / N^2 inside |k| < kmax (the per-column row runs), as col_body forms it
+  float2 f[E];
+  {
+    const int2 pr = a.prop_rows[x];
+    const float2 pxv = a.px[x];
+#pragma unroll
+    for (int c = 0; c < E; ++c) {
+      const int r = t + c * T;
+      float2 p = cmul(pxv, a.py[r]);
+      if constexpr (CONJ) p.y = -p.y;
+      const float m = (r <= pr.x || r >= pr.y) ? a.scale : 0.f;
+      f[c] = cscale(p, m);
+    }
+  }
   __syncthreads();
-  stream_tile<N>(a, item, ntl, buf0);
-  // two items per trip so each buffer plays a fixed role in each half
-  for (; item < items; item += 2 * G) {
-    stream_item<N, MODE, MUL>(a, item, item + G, items, ntl, buf0, buf1, stw);
-    const int it = item + G;
-    if (it >= items) break;
-    stream_item<N, MODE, MUL>(a, it, it + G, items, ntl, buf1, buf0, stw);
+  const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
+  auto load = [&](float2 (&v)[E], int b) {
+    const float2* src = a.src + (long long)b * a.src_bstride + x;
+#pragma unroll
+    for (int c = 0; c < E; ++c) v[c] = src[(long long)(t + c * T) * nx];
+  };
+  for (int q = 0; q < per_tile; ++q) {
+    const int b = g * per_tile + q;
+    if (b >= a.nbatch) break;  // uniform over the workgroup
+    float2 v[E];
+    load(v, b);
+    fft_line<P, N, -1, BlockSync>(v, t, stw, L);
+#pragma unroll
+    for (int c = 0; c < E; ++c) v[c] = cmul(v[c], f[c]);
+    fft_line<P, N, +1, BlockSync, LL, ZB>(v, t, stw, L);
+    float2* dst = a.dst + (long long)b * a.dst_bstride + x;
+#pragma unroll
+    for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
   }
 }
 
@@ -931,8 +965,38 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// variant 4 where its layout applies (see k_row_pl); elsewhere the best of
+// the other layouts for the length (2 at 512, 3 at 1024, 1 otherwise)
+template <int N>
+static hipError_t row_pl(int mode, int mul, const RowArgs& a, hipStream_t s) {
+  const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0) | (a.aux ? STORE_AUX : 0);
+  const bool shared_t = a.t && !a.t_index && a.t_group >= a.nbatch;
+  // a batch narrower than two waves per line would leave most lines idle
+  if (mode != ROW_INV_MUL_FWD || mul != RM_T || !shared_t || a.ny > 65535 || a.nbatch < 2 * RowPL<N>::LINES) {
+    if constexpr (N == 512) return row_nv<N, 2>(mode, mul, a, s);
+    else if constexpr (N == 1024) return row_nv<N, 3>(mode, mul, a, s);
+    else return row_nv<N, 1>(mode, mul, a, s);
+  }
+  if (a.nbatch <= 0) return hipSuccess;
+  constexpr int LINES = RowPL<N>::LINES, T = RowPL<N>::T;
+  const int per_line = std::max(1, std::min(4, (a.nbatch + LINES - 1) / LINES));
+  const int groups = (a.nbatch + LINES * per_line - 1) / (LINES * per_line);
+  if (groups > 65535) return row_nv<N, 1>(mode, mul, a, s);
+  dim3 grid(a.ny, groups), block(LINES * T);
+  if (fl == (PRUNE_IN | PRUNE_OUT)) hipLaunchKernelGGL((k_row_pl<N, PRUNE_IN | PRUNE_OUT>), grid, block, 0, s, a, per_line);
+  else if (fl == (PRUNE_IN | PRUNE_OUT | STORE_AUX))
+    hipLaunchKernelGGL((k_row_pl<N, PRUNE_IN | PRUNE_OUT | STORE_AUX>), grid, block, 0, s, a, per_line);
+  else if (fl == STORE_AUX) hipLaunchKernelGGL((k_row_pl<N, STORE_AUX>), grid, block, 0, s, a, per_line);
+  else if (fl == PRUNE_OUT) hipLaunchKernelGGL((k_row_pl<N, PRUNE_OUT>), grid, block, 0, s, a, per_line);
+  else if (fl == PRUNE_IN) hipLaunchKernelGGL((k_row_pl<N, PRUNE_IN>), grid, block, 0, s, a, per_line);
+  else if (fl == 0) hipLaunchKernelGGL((k_row_pl<N, 0>), grid, block, 0, s, a, per_line);
+  else return row_nv<N, 1>(mode, mul, a, s);
+  return hipGetLastError();
+}
+
 template <int N>
 static hipError_t row_n(int mode, int mul, const RowArgs& a, hipStream_t s, int var) {
+  if (var == 4) return row_pl<N>(mode, mul, a, s);
   if (var == 3) return row_nv<N, 3>(mode, mul, a, s);
   if (var == 2) return row_nv<N, 2>(mode, mul, a, s);
   if (var == 1) return row_nv<N, 1>(mode, mul, a, s);
@@ -976,20 +1040,14 @@ static hipError_t col_nv(int mode, int mul, const ColArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-static int device_cus() {
-  static int cus[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    cus[dev] = 256;
-  return cus[dev];
-}
-
-// persistent streamed column pass (variant 5), a full complement of resident workgroups
+// variant 5 for the propagation pass of a batch (k_col_pl); the length's
+// default layout otherwise
 template <int N>
-static hipError_t col_stream(int mode, int mul, const ColArgs& a, hipStream_t s) {
-  constexpr int C = ColStream<N>::C;
-  if (a.nbatch <= 0) return hipSuccess;
+static hipError_t col_pl(int mode, int mul, const ColArgs& a, hipStream_t s) {
+  constexpr int CV = ColPL<N>::CV, C = ColCfg<N, CV>::C;
+  const bool prop = mode == COL_FWD_MUL_INV &&
+                    (mul == CM_PROP || mul == CM_PROPCONJ || mul == CM_PROP_ZB || mul == CM_PROPCONJ_ZB);
+  if (!prop || a.nbatch < 2) return col_nv<N, CV>(mode, mul, a, s);
   if (a.nx % C != 0 || a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / ColTile0<N>::C) return hipErrorInvalidValue;
   // tiles_lo counts tiles of the default width; convert to this variant's
   ColArgs b = a;
@@ -997,35 +1055,14 @@ static hipError_t col_stream(int mode, int mul, const ColArgs& a, hipStream_t s)
     if ((a.nx / 4) % C != 0) return hipErrorInvalidValue;
     b.tiles_lo = a.nx / 4 / C;
   }
-  const int ntl = b.tiles_lo > 0 ? 2 * b.tiles_lo : a.nx / C;
-  const long long items = (long long)ntl * a.nbatch;
-  if (items > (1LL << 30)) return hipErrorInvalidValue;
-  const dim3 block(ColStream<N>::THREADS);
-  // as many resident workgroups as the LDS and registers of this instance allow
-#define MSL_COLS(M, U)                                                                                         \
-  do {                                                                                                         \
-    int per_cu = 0;                                                                                            \
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_col_stream<N, M, U>,             \
-                                                     ColStream<N>::THREADS, 0) != hipSuccess || per_cu < 1)    \
-      per_cu = 1;                                                                                              \
-    const int grid = (int)std::min<long long>(items, (long long)per_cu * device_cus());                        \
-    hipLaunchKernelGGL((k_col_stream<N, M, U>), dim3(grid), block, 0, s, b, ntl, (int)items);                  \
-  } while (0)
-  if (mode == COL_FWD) MSL_COLS(COL_FWD, CM_SCALE);
-  else if (mode == COL_INV) MSL_COLS(COL_INV, CM_SCALE);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLS(COL_FWD_MUL_INV, CM_PROP);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP_ZB) MSL_COLS(COL_FWD_MUL_INV, CM_PROP_ZB);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ_ZB) MSL_COLS(COL_FWD_MUL_INV, CM_PROPCONJ_ZB);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLS(COL_FWD_MUL_INV, CM_BAND);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLS(COL_FWD_MUL_INV, CM_GRAD);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COLS(COL_FWD_MUL_INV, CM_BANDGRAD);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL_INV, CM_SCALE);
-  else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COLS(COL_FWD_MUL, CM_PROP);
-  else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COLS(COL_FWD_MUL, CM_SCALE);
-  else if (mode == COL_MUL_INV && mul == CM_SCALE) MSL_COLS(COL_MUL_INV, CM_SCALE);
-  else return hipErrorInvalidValue;
-#undef MSL_COLS
+  const int per_tile = std::min(4, a.nbatch);
+  const int groups = (a.nbatch + per_tile - 1) / per_tile;
+  if (groups > 65535) return col_nv<N, CV>(mode, mul, a, s);
+  dim3 grid(b.tiles_lo > 0 ? 2 * b.tiles_lo : a.nx / C, groups), block(ColCfg<N, CV>::THREADS);
+  if (mul == CM_PROP) hipLaunchKernelGGL((k_col_pl<N, CM_PROP>), grid, block, 0, s, b, per_tile);
+  else if (mul == CM_PROPCONJ) hipLaunchKernelGGL((k_col_pl<N, CM_PROPCONJ>), grid, block, 0, s, b, per_tile);
+  else if (mul == CM_PROP_ZB) hipLaunchKernelGGL((k_col_pl<N, CM_PROP_ZB>), grid, block, 0, s, b, per_tile);
+  else hipLaunchKernelGGL((k_col_pl<N, CM_PROPCONJ_ZB>), grid, block, 0, s, b, per_tile);
   return hipGetLastError();
 }
 
@@ -1041,7 +1078,7 @@ static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int 
     else return col_nv<N, 0>(mode, mul, a, s);
   }
   if (var == 7) return col_nv<N, 7>(mode, mul, a, s);
-  if (var == 5) return col_stream<N>(mode, mul, a, s);
+  if (var == 5) return col_pl<N>(mode, mul, a, s);
   if (var == 4) return col_nv<N, 4>(mode, mul, a, s);
   if (var == 3) return col_nv<N, 3>(mode, mul, a, s);
   if (var == 2) return col_nv<N, 2>(mode, mul, a, s);

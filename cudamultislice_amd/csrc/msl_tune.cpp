@@ -116,7 +116,8 @@ msl_status msl_bench_pass(int kind, int variant, int n, int nbatch, int reps, fl
     if (!b.ok()) st = MSL_ERR_OOM;
     else if (b.init(s) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = MSL_ERR_HIP;
     if (st == MSL_OK && maxdiff) {
-      const int nv = std::min(2, nbatch);
+      // enough waves for the layouts that need several per workgroup
+      const int nv = std::min(n <= 1024 ? 16 : 2, nbatch);
       const size_t cnt = (size_t)nv * n * n;
       Dev A(cnt * 8), B(cnt * 8);
       std::vector<float2> ha(cnt), hb(cnt);
