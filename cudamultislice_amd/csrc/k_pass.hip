@@ -979,6 +979,8 @@ static hipError_t row_pl(int mode, int mul, const RowArgs& a, hipStream_t s) {
   }
   if (a.nbatch <= 0) return hipSuccess;
   constexpr int LINES = RowPL<N>::LINES, T = RowPL<N>::T;
+  // four waves per line: 1.129e6 probe-slice/s on the 512^2 STEM workload
+  // against 1.122e6 with eight and 1.111e6 with two
   const int per_line = std::max(1, std::min(4, (a.nbatch + LINES - 1) / LINES));
   const int groups = (a.nbatch + LINES * per_line - 1) / (LINES * per_line);
   if (groups > 65535) return row_nv<N, 1>(mode, mul, a, s);
