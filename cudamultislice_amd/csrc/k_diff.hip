@@ -19,8 +19,10 @@ namespace msl {
 
 // One line per T threads; lines never straddle a wavefront when T <= 64, so
 // the FFT exchanges synchronise the wave only (padded LDS, as the row pass).
-// At 1024 the pass holds 216 VGPRs (two waves per SIMD): two four-line
-// workgroups per CU, and a 1024-row grid fills the 512 slots twice.
+// At 1024 the stored-wave pass holds 235 VGPRs and the one that transforms z
+// 247 (two waves per SIMD): two four-line workgroups per CU, and a 1024-row
+// grid fills the 512 slots twice.  Both variants use the same line count
+// (adj_lines takes the variant so the host need not assume it).
 template <int N, bool REAL_Z> struct AdjCfg {
   static constexpr int T = Plan<N>::T;
   static constexpr bool WAVE = T <= 64;
