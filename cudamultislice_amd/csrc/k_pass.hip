@@ -525,9 +525,13 @@ __device__ __forceinline__ void col_block(const A& a, int bx, int b, int nbx, fl
   const long long nx = a.nx;
   float2 v[E];
   {
-    const float2* src = a.src + (long long)b * a.src_bstride + x;
+    // the wave's base is uniform (a scalar register) and the offsets within
+    // one wave fit 32 bits: one vector add per access instead of a 64-bit
+    // multiply-add
+    const float2* src = a.src + (long long)b * a.src_bstride;
+    const unsigned o0 = (unsigned)t * (unsigned)nx + (unsigned)x, step = (unsigned)T * (unsigned)nx;
 #pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = src[(long long)(t + c * T) * nx];
+    for (int c = 0; c < E; ++c) v[c] = src[o0 + (unsigned)c * step];
   }
   using LL = LdsLine<Cfg::SH>;
   const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
@@ -562,9 +566,10 @@ __device__ __forceinline__ void col_block(const A& a, int bx, int b, int nbx, fl
     if (a.dst == nullptr) return;
   }
   {
-    float2* dst = a.dst + (long long)b * a.dst_bstride + x;
+    float2* dst = a.dst + (long long)b * a.dst_bstride;
+    const unsigned o0 = (unsigned)t * (unsigned)nx + (unsigned)x, step = (unsigned)T * (unsigned)nx;
 #pragma unroll
-    for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
+    for (int c = 0; c < E; ++c) dst[o0 + (unsigned)c * step] = v[c];
   }
 }
 
@@ -717,10 +722,11 @@ __global__ void __launch_bounds__((ColCfg<N, ColPL<N>::CV>::THREADS), 4) k_col_p
   }
   __syncthreads();
   const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
+  const unsigned o0 = (unsigned)t * (unsigned)nx + (unsigned)x, step = (unsigned)T * (unsigned)nx;
   auto load = [&](float2 (&v)[E], int b) {
-    const float2* src = a.src + (long long)b * a.src_bstride + x;
+    const float2* src = a.src + (long long)b * a.src_bstride;
 #pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = src[(long long)(t + c * T) * nx];
+    for (int c = 0; c < E; ++c) v[c] = src[o0 + (unsigned)c * step];
   };
   for (int q = 0; q < per_tile; ++q) {
     const int b = g * per_tile + q;
@@ -731,9 +737,9 @@ __global__ void __launch_bounds__((ColCfg<N, ColPL<N>::CV>::THREADS), 4) k_col_p
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = cmul(v[c], f[c]);
     fft_line<P, N, +1, BlockSync, LL, ZB>(v, t, stw, L);
-    float2* dst = a.dst + (long long)b * a.dst_bstride + x;
+    float2* dst = a.dst + (long long)b * a.dst_bstride;
 #pragma unroll
-    for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
+    for (int c = 0; c < E; ++c) dst[o0 + (unsigned)c * step] = v[c];
   }
 }
 
