@@ -59,8 +59,10 @@ static int lg2(int n) {
 //         5 a workgroup per tile looping over waves, propagator factors in
 //           registers (k_col_pl; the propagation pass only, the length's
 //           default elsewhere)
-//         6 the tile staged through LDS, then one column per wave with
-//           wave-local exchanges (two workgroup barriers per tile)
+//         6 as 7 with each column held by consecutive threads of one wave
+//           (wave-local exchanges, no workgroup barriers; narrower row
+//           segments per load instruction).  It replaced a layout that
+//           staged the tile through LDS for the same purpose (slower).
 //         7 as 2 with half as many columns per workgroup
 // (a staged column layout with one wavefront per column and an LDS pad of
 // 1/8 were measured slower or equal on MI355X and were removed)
@@ -88,10 +90,14 @@ int num_pass_variants(int kind) { return kind == 0 ? 5 : 8; }
 template <int N> struct ColTile0 { static constexpr int C = (N >= 2048) ? 4 : (N >= 1024 ? 8 : 16); };
 
 template <int N, int CV> struct ColCfg {
-  using P = typename std::conditional<CV == 2 || CV == 7, AltPlan<N>, Plan<N>>::type;
-  static constexpr bool INTERLEAVED = CV != 1;
+  using P = typename std::conditional<CV == 2 || CV == 6 || CV == 7, AltPlan<N>, Plan<N>>::type;
+  // variant 6: consecutive threads of a wave hold one column (T <= 64), so the
+  // exchanges synchronise the wave only; loads take C / (64 / T) columns'
+  // worth of each row per instruction instead of the whole tile row
+  static constexpr bool WAVECOL = CV == 6 && P::T <= 64 && 64 % P::T == 0;
+  static constexpr bool INTERLEAVED = CV != 1 && !WAVECOL;
   static constexpr int T = P::T;
-  static constexpr int C = (CV == 3 || CV == 7) ? ColTile0<N>::C / 2 : ColTile0<N>::C;
+  static constexpr int C = (CV == 3 || CV == 6 || CV == 7) ? ColTile0<N>::C / 2 : ColTile0<N>::C;
   static constexpr int SH = INTERLEAVED ? 0 : 4;
   // [col][row] line stride in float2: span + padding so consecutive columns
   // start two banks apart (stride = 2 mod 32 float2)
@@ -518,7 +524,9 @@ __device__ __forceinline__ void col_body(float2 (&v)[N / P::T], int t, int x, co
 template <int N, int MODE, int MUL, class Cfg, class A>
 __device__ __forceinline__ void col_block(const A& a, int bx, int b, int nbx, float2* lds, const float2* stw) {
   constexpr int T = Cfg::T, E = N / T, C = Cfg::C;
-  const int col = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = Cfg::WAVECOL ? threadIdx.x / T : threadIdx.x % C;
+  const int t = Cfg::WAVECOL ? threadIdx.x % T : threadIdx.x / C;
+  using S = typename std::conditional<Cfg::WAVECOL, WaveSync, BlockSync>::type;
   // band-limited runs launch only the tiles at both ends of kx
   const int tile = (a.tiles_lo > 0 && bx >= a.tiles_lo) ? bx + a.nx / C - 2 * a.tiles_lo : bx;
   const int x = tile * C + col;
@@ -537,9 +545,9 @@ __device__ __forceinline__ void col_block(const A& a, int bx, int b, int nbx, fl
   const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
   if constexpr (Cfg::DUAL) {
     const DualLine<LL> D{L, LL{lds + N * C + col, C}};
-    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, D, stw);
+    col_body<typename Cfg::P, N, MODE, MUL, S>(v, t, x, a, D, stw);
   } else {
-    col_body<typename Cfg::P, N, MODE, MUL, BlockSync>(v, t, x, a, L, stw);
+    col_body<typename Cfg::P, N, MODE, MUL, S>(v, t, x, a, L, stw);
   }
   if constexpr (MODE == COL_FWD_DETECT) {
     const float kxv = a.kx[x];
@@ -583,99 +591,6 @@ __global__ void __launch_bounds__((ColCfg<N, CV>::THREADS), (ColCfg<N, CV>::MIN_
   // logical (tile, wave) of this workgroup, XCD-aware
   const long long gid = xcd_remap((long long)blockIdx.y * gridDim.x + blockIdx.x, (long long)gridDim.x * gridDim.y);
   col_block<N, MODE, MUL, Cfg>(a, (int)(gid % gridDim.x), (int)(gid / gridDim.x), (int)gridDim.x, lds, stw);
-}
-
-// ------------------------------------------------------------------ wave-local columns
-// Variant 6: the tile of C columns is read row by row (coalesced) into LDS,
-// one column per LDS line ([col][row], padding 1/16); after one workgroup
-// barrier each wave transforms its own column(s) with wave-local exchanges in
-// that line (no workgroup barriers inside the transforms), writes the result
-// back to the line, and after a second barrier the tile is stored row by row.
-// Lengths 128..1024 (the plan's T <= 64 keeps a column inside one wave).
-template <int N> struct ColWL {
-  using P = Plan<N>;
-  static constexpr int T = P::T, E = N / T;
-  static constexpr int C = N >= 512 ? 8 : 16;  // columns per workgroup
-  static constexpr int THREADS = C * T, SH = 4;
-  static constexpr int LS = LdsLine<SH>::span(N) + 1;
-  static_assert(T <= 64 && 64 % T == 0, "a column must stay inside one wave");
-};
-
-template <int N, int MODE, int MUL>
-__global__ void __launch_bounds__(ColWL<N>::THREADS) k_col_wl(ColArgs a) {
-  using W = ColWL<N>;
-  using LL = LdsLine<W::SH>;
-  constexpr int T = W::T, E = W::E, C = W::C;
-  __shared__ float2 lds[C * W::LS];
-  __shared__ float2 stw[N];
-  for (int i = threadIdx.x; i < N; i += W::THREADS) stw[i] = a.tw[i];
-  const long long gid = xcd_remap((long long)blockIdx.y * gridDim.x + blockIdx.x, (long long)gridDim.x * gridDim.y);
-  const int bx = (int)(gid % gridDim.x), b = (int)(gid / gridDim.x);
-  const int tile = (a.tiles_lo > 0 && bx >= a.tiles_lo) ? bx + a.nx / C - 2 * a.tiles_lo : bx;
-  const int x0 = tile * C;
-  const long long nx = a.nx;
-  // coalesced load: thread (cc, r0) reads rows r0 + k T of column x0 + cc
-  {
-    const int cc = threadIdx.x % C, r0 = threadIdx.x / C;
-    const float2* src = a.src + (long long)b * a.src_bstride + x0 + cc;
-    float2 u[E];
-#pragma unroll
-    for (int k = 0; k < E; ++k) u[k] = src[(long long)(r0 + k * T) * nx];
-    const LL L{lds + cc * W::LS, 1};
-#pragma unroll
-    for (int k = 0; k < E; ++k) L.at(r0 + k * T) = u[k];
-  }
-  __syncthreads();
-  const int col = threadIdx.x / T, t = threadIdx.x % T;
-  const LL L{lds + col * W::LS, 1};
-  float2 v[E];
-#pragma unroll
-  for (int c = 0; c < E; ++c) v[c] = L.at(t + c * T);
-  WaveSync::sync();  // the wave's reads of its line are done before the exchanges rewrite it
-  col_body<typename W::P, N, MODE, MUL, WaveSync>(v, t, x0 + col, a, L, stw);
-  WaveSync::sync();
-#pragma unroll
-  for (int c = 0; c < E; ++c) L.at(t + c * T) = v[c];
-  __syncthreads();
-  {
-    const int cc = threadIdx.x % C, r0 = threadIdx.x / C;
-    const LL Ls{lds + cc * W::LS, 1};
-    float2* dst = a.dst + (long long)b * a.dst_bstride + x0 + cc;
-#pragma unroll
-    for (int k = 0; k < E; ++k) dst[(long long)(r0 + k * T) * nx] = Ls.at(r0 + k * T);
-  }
-}
-
-template <int N>
-static hipError_t col_wl(int mode, int mul, const ColArgs& a, hipStream_t s) {
-  using W = ColWL<N>;
-  constexpr int C = W::C;
-  if (a.nbatch <= 0) return hipSuccess;
-  if (a.nx % C != 0 || a.ny != N) return hipErrorInvalidValue;
-  if (a.tiles_lo < 0 || 2 * a.tiles_lo > a.nx / ColTile0<N>::C) return hipErrorInvalidValue;
-  ColArgs b = a;
-  if (a.tiles_lo > 0) {
-    if ((a.nx / 4) % C != 0) return hipErrorInvalidValue;
-    b.tiles_lo = a.nx / 4 / C;
-  }
-  const dim3 grid(b.tiles_lo > 0 ? 2 * b.tiles_lo : a.nx / C, a.nbatch), block(W::THREADS);
-#define MSL_COLW(M, U) hipLaunchKernelGGL((k_col_wl<N, M, U>), grid, block, 0, s, b)
-  if (mode == COL_FWD) MSL_COLW(COL_FWD, CM_SCALE);
-  else if (mode == COL_INV) MSL_COLW(COL_INV, CM_SCALE);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP) MSL_COLW(COL_FWD_MUL_INV, CM_PROP);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ) MSL_COLW(COL_FWD_MUL_INV, CM_PROPCONJ);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROP_ZB) MSL_COLW(COL_FWD_MUL_INV, CM_PROP_ZB);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_PROPCONJ_ZB) MSL_COLW(COL_FWD_MUL_INV, CM_PROPCONJ_ZB);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_BAND) MSL_COLW(COL_FWD_MUL_INV, CM_BAND);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_GRAD) MSL_COLW(COL_FWD_MUL_INV, CM_GRAD);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_BANDGRAD) MSL_COLW(COL_FWD_MUL_INV, CM_BANDGRAD);
-  else if (mode == COL_FWD_MUL_INV && mul == CM_SCALE) MSL_COLW(COL_FWD_MUL_INV, CM_SCALE);
-  else if (mode == COL_FWD_MUL && mul == CM_PROP) MSL_COLW(COL_FWD_MUL, CM_PROP);
-  else if (mode == COL_FWD_MUL && mul == CM_SCALE) MSL_COLW(COL_FWD_MUL, CM_SCALE);
-  else if (mode == COL_MUL_INV && mul == CM_SCALE) MSL_COLW(COL_MUL_INV, CM_SCALE);
-  else return hipErrorInvalidValue;
-#undef MSL_COLW
-  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ columns looping over waves
@@ -722,11 +637,12 @@ __global__ void __launch_bounds__((ColCfg<N, ColPL<N>::CV>::THREADS), 4) k_col_p
   }
   __syncthreads();
   const LL L = Cfg::INTERLEAVED ? LL{lds + col, C} : LL{lds + col * Cfg::LSC, 1};
-  const unsigned o0 = (unsigned)t * (unsigned)nx + (unsigned)x, step = (unsigned)T * (unsigned)nx;
+  // (64-bit addressing here: the 32-bit offsets of col_block, held across
+  // the loop, spill under the four-wave register budget)
   auto load = [&](float2 (&v)[E], int b) {
-    const float2* src = a.src + (long long)b * a.src_bstride;
+    const float2* src = a.src + (long long)b * a.src_bstride + x;
 #pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = src[o0 + (unsigned)c * step];
+    for (int c = 0; c < E; ++c) v[c] = src[(long long)(t + c * T) * nx];
   };
   for (int q = 0; q < per_tile; ++q) {
     const int b = g * per_tile + q;
@@ -737,9 +653,9 @@ __global__ void __launch_bounds__((ColCfg<N, ColPL<N>::CV>::THREADS), 4) k_col_p
 #pragma unroll
     for (int c = 0; c < E; ++c) v[c] = cmul(v[c], f[c]);
     fft_line<P, N, +1, BlockSync, LL, ZB>(v, t, stw, L);
-    float2* dst = a.dst + (long long)b * a.dst_bstride;
+    float2* dst = a.dst + (long long)b * a.dst_bstride + x;
 #pragma unroll
-    for (int c = 0; c < E; ++c) dst[o0 + (unsigned)c * step] = v[c];
+    for (int c = 0; c < E; ++c) dst[(long long)(t + c * T) * nx] = v[c];
   }
 }
 
@@ -752,7 +668,7 @@ __global__ void __launch_bounds__((ColCfg<N, ColPL<N>::CV>::THREADS), 4) k_col_p
 template <int N> struct GroupRow { static constexpr int RV = N == 512 ? 2 : 1; };
 template <int N> struct GroupCol {
   using P = Plan<N>;
-  static constexpr bool INTERLEAVED = true, DUAL = false;
+  static constexpr bool INTERLEAVED = true, DUAL = false, WAVECOL = false;
   static constexpr int T = P::T, THREADS = 512, C = THREADS / T, SH = 0, LSC = 0, LDS = N * C;
   static_assert(C * T == THREADS && C <= N / 4, "tile does not fit the band");
 };
@@ -1082,7 +998,7 @@ static hipError_t col_n(int mode, int mul, const ColArgs& a, hipStream_t s, int 
     return var == 2 ? col_nv<N, 2>(mode, mul, a, s) : col_nv<N, 0>(mode, mul, a, s);
   }
   if (var == 6) {
-    if constexpr (N >= 128 && N <= 1024) return col_wl<N>(mode, mul, a, s);
+    if constexpr (ColCfg<N, 6>::WAVECOL) return col_nv<N, 6>(mode, mul, a, s);
     else return col_nv<N, 0>(mode, mul, a, s);
   }
   if (var == 7) return col_nv<N, 7>(mode, mul, a, s);
