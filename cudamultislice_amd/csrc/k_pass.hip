@@ -273,8 +273,13 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
       long long slot = a.t_index ? (long long)a.t_index[b0] : b0 / a.t_group;
       if (db != 0) slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
+      if (a.scale == 1.f) {  // every caller: no scaling multiplies
 #pragma unroll
-      for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
+        for (int c = 0; c < E; ++c) v[c] = cmul(v[c], tr[t + c * T]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
+      }
     }
   } else {
     if (a.scale != 1.0f) {
@@ -359,6 +364,12 @@ __global__ void __launch_bounds__((RowPL<N>::LINES * RowPL<N>::T)) k_row_pl(RowA
     const float2* trow = a.t + (long long)y * N;
 #pragma unroll
     for (int c = 0; c < E; ++c) tr[c] = trow[t + c * T];
+    // the scale goes into the factors once (every caller passes 1: the
+    // products are those of the unscaled form)
+    if (a.scale != 1.f) {
+#pragma unroll
+      for (int c = 0; c < E; ++c) tr[c] = cscale(tr[c], a.scale);
+    }
   }
   __syncthreads();
   const LL L{lds + ln * LS, 1};
@@ -389,7 +400,7 @@ __global__ void __launch_bounds__((RowPL<N>::LINES * RowPL<N>::T)) k_row_pl(RowA
       }
     }
 #pragma unroll
-    for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[c]), a.scale);
+    for (int c = 0; c < E; ++c) v[c] = cmul(v[c], tr[c]);
     fft_line<P, N, -1, S>(v, t, stw, L);
     if (b < a.nbatch) {
       float2* row = a.dst + (long long)b * a.dst_bstride + (long long)y * N;
