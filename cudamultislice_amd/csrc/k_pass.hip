@@ -273,13 +273,8 @@ __device__ __forceinline__ void row_block(const A& a, long long blk, float2* lds
       long long slot = a.t_index ? (long long)a.t_index[b0] : b0 / a.t_group;
       if (db != 0) slot = a.t_index ? (long long)a.t_index[b] : (long long)(b / a.t_group);
       const float2* tr = a.t + slot * a.t_bstride + (long long)y * N;
-      if (a.scale == 1.f) {  // every caller: no scaling multiplies
 #pragma unroll
-        for (int c = 0; c < E; ++c) v[c] = cmul(v[c], tr[t + c * T]);
-      } else {
-#pragma unroll
-        for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
-      }
+      for (int c = 0; c < E; ++c) v[c] = cscale(cmul(v[c], tr[t + c * T]), a.scale);
     }
   } else {
     if (a.scale != 1.0f) {
