@@ -177,6 +177,39 @@ def _diff_time(eng, Vt, p0, target, reps):
     return tf * 1e3, tfb * 1e3, eng.info()["workspace_bytes"]
 
 
+def copy_rates_gbs(nbytes=4 << 30, reps=5):
+    """bytes read + written per second by two device-to-device copies of
+    nbytes: torch's copy_ and libmsl's streaming copy kernel (16-byte vector
+    loads and stores, grid-stride), each timed with HIP events on torch's
+    current stream, median of reps.  msl_copy_device waits for its kernel, so
+    its interval also holds the host's return (microseconds against ~1 ms)."""
+    import ctypes
+    import torch
+    from cudamultislice_amd._lib import lib
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ops = {"torch_copy": lambda: dst.copy_(src),
+           "k_copy16": lambda: lib().msl_copy_device(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()),
+                                                    ctypes.c_longlong(nbytes), st)}
+    out = {}
+    for name, op in ops.items():
+        op()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            r = op()
+            e1.record()
+            e1.synchronize()
+            assert name != "k_copy16" or r == 0
+            ts.append(e0.elapsed_time(e1))
+        out[name] = 2 * nbytes / (sorted(ts)[len(ts) // 2] * 1e-3) / 1e9
+    del src, dst
+    return out
+
+
 def tem_cases(cpu_threads):
     """Plane-wave TEM through the reference-shaped API: (1) Si 4x4x10 cells,
     256^2 over 21.72 A, 1 A slices, 200 kV, exit wave + CTF image (df -500 A,
@@ -470,6 +503,31 @@ def main():
             "launch_timing": f"HIP events around each launch on the handle's stream, {ks[kern]['launches']} "
                              f"eager launches over {max(1, args.profile_configs)} configurations after the timed region",
             "other_passes": {k: ks[k]["ms_avg"] for k in ("row", "column", "detector", "potential")}}
+    # what plain device copies reach on this GPU (read + write, 4 GiB each
+    # way): the practical HBM rate next to the 8 TB/s peak.  The passes'
+    # algorithmic rates can exceed it where the Infinity Cache serves part of
+    # their bytes (256-probe launches move 256 MiB of in-band waves).
+    cps = copy_rates_gbs()
+    cp = max(cps.values())
+    roof["copy_GBps"] = {k: round(v, 1) for k, v in cps.items()}
+    roof["frac_of_copy"] = round(achieved / cp, 4)
+    col_rate = int(16 * B * npix * band) / (ks["column"]["ms_avg"] * 1e-3) / 1e9
+    roof["column_GBps"] = round(col_rate, 1)
+    log(f"[bench] device copies {roof['copy_GBps']} GB/s (read + write); {kern} pass at {achieved / cp:.3f} "
+        f"of the faster, column pass {col_rate:.0f} GB/s")
+    # the row pass is bound by vector-ALU issue: instructions per launch from
+    # the committed counter run (per wave, scaled to this batch) against one
+    # wave64 instruction per SIMD every 4 cycles at 2.4 GHz
+    vpath = os.path.join(ROOT, "profiles", "valu_stem512.json")
+    if os.path.exists(vpath) and band < 1.0:
+        vk = json.load(open(vpath))["kernels"].get(kern)
+        if vk:
+            instr = vk["valu_instr_per_wave"] * B
+            floor_ms = instr * 4 / (1024 * 2.4e9) * 1e3
+            roof["valu"] = {"instr_per_launch": instr, "min_ms_at_issue_peak": round(floor_ms, 5),
+                            "issue_frac": round(floor_ms / ks[kern]["ms_avg"], 4),
+                            "source": "profiles/valu_stem512.json (rocprofv3 SQ_INSTS_VALU per wave)"}
+            log(f"[bench] {kern} pass VALU issue {floor_ms / ks[kern]['ms_avg']:.3f} of the 2.4 GHz rate")
 
     per_probe = None
     if args.per_probe > 0:
