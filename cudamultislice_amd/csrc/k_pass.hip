@@ -76,9 +76,12 @@ static int lg2(int n) {
 // vs 0.51 ms per 1024 waves for half-width tiles on the default plan, 1 %
 // more probe-slices per second on the 512^2 STEM workload); 256, 1024 and
 // 2048 variant 0 (1024: 0.54 vs 0.62 ms per 256 waves for the half-width
-// tiles once the inverse skipped the propagator's zero rows).
-static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 4, 4, 1};
-static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 7, 0, 0};
+// tiles once the inverse skipped the propagator's zero rows).  At 2048 the
+// loops over waves won once they existed: rows variant 4 (0.447-0.469 vs
+// 0.525-0.531 ms per 32 waves for variant 1), columns variant 5 (0.376-0.378
+// vs 0.440-0.447 ms for variant 0); at 256 variants 1 and 0 stay fastest.
+static int g_row_var[12] = {0, 0, 0, 0, 1, 1, 1, 1, 1, 4, 4, 4};
+static int g_col_var[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 7, 0, 5};
 
 int pass_variant(int kind, int n) { return kind == 0 ? g_row_var[lg2(n)] : g_col_var[lg2(n)]; }
 void set_pass_variant(int kind, int n, int v) {
