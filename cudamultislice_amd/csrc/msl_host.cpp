@@ -1533,14 +1533,23 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
     ~Graph() { reset(); }
   } graph;
   double tb = 0, tp = 0, tprop = 0;
-  std::vector<float> dx, dy, dzv;
+  // displacements of the configuration being binned, and the next
+  // configuration's, drawn on the host while this one propagates (the GPU
+  // would otherwise wait for them between configurations)
+  std::vector<float> dx, dy, dzv, ndx, ndy, ndz;
+  bool have_next = false;
   for (int cfg = 0; cfg < nconf; ++cfg) {
     require(s->cancel == 0, MSL_ERR_CANCELLED, "cancelled");
     double a0 = now_ms();
     if (displace) {
       const int n = (int)s->x.size();
-      dx.resize(n); dy.resize(n); dzv.resize(n);
-      tds_disp(d->seed, cfg, n, d->sigma, dx.data(), dy.data(), dzv.data());
+      if (have_next) {
+        dx.swap(ndx); dy.swap(ndy); dzv.swap(ndz);
+        have_next = false;
+      } else {
+        dx.resize(n); dy.resize(n); dzv.resize(n);
+        tds_disp(d->seed, cfg, n, d->sigma, dx.data(), dy.data(), dzv.data());
+      }
       bin_atoms(s, dx.data(), dy.data(), dzv.data());
     } else if (cfg == 0 || !s->binned) {
       bin_atoms(s, nullptr, nullptr, nullptr);
@@ -1581,6 +1590,12 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
       } else {
         enqueue_chunk(nb, &kt);
         kt.flush();
+      }
+      if (displace && p0 == 0 && cfg + 1 < nconf) {
+        const int n = (int)s->x.size();
+        ndx.resize(n); ndy.resize(n); ndz.resize(n);
+        tds_disp(d->seed, cfg + 1, n, d->sigma, ndx.data(), ndy.data(), ndz.data());
+        have_next = true;
       }
       if (want_exit) {
         RowArgs ri = row_args(s, W.p, W.p, nb, np);

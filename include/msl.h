@@ -227,7 +227,9 @@ msl_status msl_autotune_batch(msl_sim* sim, const msl_stem_batch_desc* desc, int
                               float* us_per_probe_slice, int* ncand);
 
 /* time spent in the last msl_stem_batched, split by stage (milliseconds):
- * [0] binning, [1] potentials, [2] propagation, [3] total */
+ * [0] binning, [1] potentials, [2] propagation, [3] total.  The frozen-phonon
+ * displacements of every configuration after the first are drawn on the host
+ * while the previous configuration propagates, inside [2]. */
 msl_status msl_last_timings(const msl_sim* sim, float* out4);
 /* per-kernel timing of batched runs with HIP events on the handle's stream
  * (enabling resets the counters); out8 = {row ms, row launches, column ms,
