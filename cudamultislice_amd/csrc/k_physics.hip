@@ -491,9 +491,14 @@ hipError_t launch_potential_group(const PotArgs* tab, const long long* blk0, int
 // k < apert.  theta = 2 pi (kx posx + ky posy) ps is reduced in FP64 so the
 // ramp stays accurate for positions far from the origin.
 __global__ void k_probe_kspace(ProbeArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  const int b = blockIdx.z;
-  if (x >= a.nx) return;
+  int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, b = blockIdx.z;
+  if (a.band_cols > 0) {
+    if (x >= 2 * a.band_cols) return;
+    if (x >= a.band_cols) x += a.nx - 2 * a.band_cols;
+  } else if (x >= a.nx) {
+    return;
+  }
   const float k0x = a.kx[x], k0y = a.ky[y];
   const float k = sqrtf(k0x * k0x + k0y * k0y);
   float2 v = make_float2(0.f, 0.f);
@@ -512,6 +517,7 @@ __global__ void k_probe_kspace(ProbeArgs a) {
 }
 
 hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s) {
+  if (a.band_cols < 0 || 2 * a.band_cols > a.nx) return hipErrorInvalidValue;
   // probes on grid.z, at most 65535 per launch
   constexpr int kMaxGridZ = 65535;
   for (int b0 = 0; b0 < a.nprobe; b0 += kMaxGridZ) {
@@ -522,7 +528,8 @@ hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s) {
       c.posx = a.posx + b0;
       c.posy = a.posy + b0;
     }
-    hipLaunchKernelGGL(k_probe_kspace, dim3(nblk(a.nx, 256), a.ny, c.nprobe), dim3(256), 0, s, c);
+    const int cols = a.band_cols > 0 ? 2 * a.band_cols : a.nx;
+    hipLaunchKernelGGL(k_probe_kspace, dim3(nblk(cols, 256), a.ny, c.nprobe), dim3(256), 0, s, c);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

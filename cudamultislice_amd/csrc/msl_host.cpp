@@ -1467,6 +1467,17 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
   require(norm > 0 && std::isfinite(norm), MSL_ERR_RANGE, "probe has zero amplitude (aperture too small?)");
   const bool want_exit = d->exit_out != nullptr;
   const bool prune = s->prune_tiles > 0;
+  // The probes' spectra vanish outside the aperture (k_probe_kspace's FP32
+  // test k < apert, and k >= |kx|): when every column the band pruning drops
+  // lies outside it, by a margin, probe synthesis, the column IFFT and the
+  // first row pass move only the band's columns.  The others are then never
+  // written, and no pass reads them (the |Psi|^2 map, which reads every
+  // column, keeps the full set-up).
+  const float apert = (float)((double)s->obj * 0.001 / (double)s->wavel);
+  bool probe_in_band = prune && !d->pacbed_out;
+  for (int x = nx / 4; x < 3 * nx / 4 && probe_in_band; ++x)
+    if (!(std::sqrt(s->hkx[x] * s->hkx[x]) > apert * 1.0001f)) probe_in_band = false;
+  const int band_cols = probe_in_band ? nx / 4 : 0;
   // detector tiles per probe: the |Psi|^2 map needs every column written
   const int det_tiles = (prune && !d->pacbed_out) ? 2 * s->prune_tiles : ntiles;
   // One chunk of nb probes whose positions are in pos: probe synthesis, the
@@ -1478,18 +1489,21 @@ extern "C" msl_status msl_stem_batched(msl_sim* s, const msl_stem_batch_desc* d)
     ProbeArgs p{};
     p.out = W.p; p.bstride = np; p.nx = nx; p.ny = ny; p.nprobe = nb; p.kx = s->kx.p; p.ky = s->ky.p;
     p.posx = pos.p; p.posy = pos.p + B;
-    p.apert = (float)((double)s->obj * 0.001 / (double)s->wavel);
+    p.apert = apert;
     p.ps = s->ps; p.df = s->df; p.Cs = s->Cs; p.wavel = s->wavel;
+    p.band_cols = band_cols;
     HIPC(launch_probe_kspace(p, s->stream));
     ColArgs ci = col_args(s, W.p, W.p, nb, np);
     ci.scale = 1.0f / norm;
+    if (probe_in_band) ci.tiles_lo = s->prune_tiles;
     col(s, COL_INV, CM_SCALE, ci);
     for (int i = 1; i <= nslices; ++i) {
       RowArgs r = row_args(s, W.p, W.p, nb, np);
       r.t = tall.p + (size_t)(i - 1) * np;
       r.t_group = nb;
       r.t_bstride = 0;
-      r.prune_in = prune && i > 1;  // the probe itself is not band-limited
+      // the probe is band-limited when its aperture lies inside the band
+      r.prune_in = prune && (i > 1 || probe_in_band);
       r.prune_out = prune;
       if (kt) kt->begin(0);
       row(s, ROW_INV_MUL_FWD, RM_T, r);

@@ -222,6 +222,10 @@ struct ProbeArgs {
   const int* posy;
   int px0, py0;
   float apert, ps, df, Cs, wavel;
+  // > 0: write only the columns x < band_cols and x >= nx - band_cols (the
+  // caller has checked that the aperture lies inside them; the other
+  // columns are left as they are)
+  int band_cols;
 };
 hipError_t launch_probe_kspace(const ProbeArgs& a, hipStream_t s);
 
