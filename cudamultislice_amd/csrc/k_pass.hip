@@ -48,7 +48,8 @@ static int lg2(int n) {
 //           threads per line)
 //         3 as 1 with 512-thread workgroups (twice the lines)
 //         4 a workgroup per row looping over waves, transmission in registers
-//           (k_row_pl; IFFT -> shared t -> FFT only, variant 2 elsewhere)
+//           (k_row_pl; IFFT -> t -> FFT only, the best of the others
+//           elsewhere)
 // column: 0 [row][col] interleaved LDS, workgroup barriers
 //         1 [col][row] LDS with pad 1/16, workgroup barriers
 //         2 as 0 with the alternative plan
@@ -172,7 +173,9 @@ template <int N, int RV> struct RowCfg {
 // them (the column pass never reads the others).
 // STORE_AUX: the pass also writes the real-space wave of ROW_INV_MUL_FWD
 // (the adjoint reuses it instead of transforming again).
-enum { PRUNE_IN = 1, PRUNE_OUT = 2, STORE_AUX = 4 };
+// T_SLOTS (row loop only): each wave reads the transmission of its own slot
+// (RowArgs::t_index or b / t_group) instead of one shared by the launch
+enum { PRUNE_IN = 1, PRUNE_OUT = 2, STORE_AUX = 4, T_SLOTS = 8 };
 template <int E> struct BandBlocks {
   static constexpr int LO = E / 4 > 0 ? E / 4 : 1;
   static __host__ __device__ constexpr bool active(int c) { return c < LO || c >= E - LO; }
@@ -330,12 +333,14 @@ __global__ void __launch_bounds__((RowCfg<N, RV>::LINES * RowCfg<N, RV>::T)) k_r
   row_block<N, MODE, MUL, FL, RV>(a, blk, lds, stw);
 }
 
-// Variant 4, for IFFT_x -> t -> FFT_x with one transmission shared by every
-// wave of the launch (a batched STEM chunk, the differentiable forward): a
-// workgroup owns row y of LINES x per_line waves.  Each thread keeps its E
-// transmission values in registers for all of them (the other layouts load
-// them once per wave), and requests the next wave's row before transforming
-// the current one.
+// Variant 4, for IFFT_x -> t -> FFT_x: a workgroup owns row y of LINES x
+// per_line waves.  Each thread keeps its E transmission values in registers
+// for all of them (the other layouts load them once per wave), and requests
+// the next wave's row before transforming the current one.  With one
+// transmission shared by the launch (a batched STEM chunk, the
+// differentiable forward) they are loaded once; with per-wave slots
+// (T_SLOTS: the serving engine's batches of several jobs, from 256 up) a
+// line reloads them when its next wave reads another slot.
 template <int N> struct RowPL {
   using P = typename std::conditional<(N <= 512), AltPlan<N>, Plan<N>>::type;
   static constexpr int T = P::T;
@@ -357,9 +362,16 @@ __global__ void __launch_bounds__((RowPL<N>::LINES * RowPL<N>::T)) k_row_pl(RowA
   const int tid = threadIdx.x, t = tid % T, ln = tid / T;
   for (int i = tid; i < N; i += NT) stw[i] = a.tw[i];
   const int y = blockIdx.x, g = blockIdx.y;
+  // lines past the batch read wave 0 and store nothing
+  auto wave_of = [&](int q) { return (g * per_line + q) * LINES + ln; };
+  // transmission slot of wave b (T_SLOTS; otherwise the launch's one row)
+  auto slot_of = [&](int b) -> long long {
+    const int bb = b < a.nbatch ? b : 0;
+    return a.t_index ? (long long)a.t_index[bb] : (long long)(bb / a.t_group);
+  };
   float2 tr[E];
-  {
-    const float2* trow = a.t + (long long)y * N;
+  auto load_t = [&](long long slot) {
+    const float2* trow = a.t + slot * a.t_bstride + (long long)y * N;
 #pragma unroll
     for (int c = 0; c < E; ++c) tr[c] = trow[t + c * T];
     // the scale goes into the factors once (every caller passes 1: the
@@ -368,12 +380,12 @@ __global__ void __launch_bounds__((RowPL<N>::LINES * RowPL<N>::T)) k_row_pl(RowA
 #pragma unroll
       for (int c = 0; c < E; ++c) tr[c] = cscale(tr[c], a.scale);
     }
-  }
+  };
+  long long cur = (FL & T_SLOTS) ? slot_of(wave_of(0)) : 0;
+  load_t(cur);
   __syncthreads();
   const LL L{lds + ln * LS, 1};
   const float2 zero = make_float2(0.f, 0.f);
-  // lines past the batch read wave 0 and store nothing
-  auto wave_of = [&](int q) { return (g * per_line + q) * LINES + ln; };
   auto load = [&](float2 (&v)[E], int b) {
     const float2* row = a.src + (long long)(b < a.nbatch ? b : 0) * a.src_bstride + (long long)y * N;
 #pragma unroll
@@ -395,6 +407,14 @@ __global__ void __launch_bounds__((RowPL<N>::LINES * RowPL<N>::T)) k_row_pl(RowA
         float2* row = a.aux + (long long)b * a.dst_bstride + (long long)y * N;
 #pragma unroll
         for (int c = 0; c < E; ++c) nt_store(v[c], &row[t + c * T]);
+      }
+    }
+    if constexpr ((FL & T_SLOTS) != 0) {
+      // a line's waves lie LINES apart: a new job's wave brings its row
+      const long long sb = slot_of(b);
+      if (sb != cur) {
+        cur = sb;
+        load_t(sb);
       }
     }
 #pragma unroll
@@ -900,10 +920,15 @@ static hipError_t row_nv(int mode, int mul, const RowArgs& a, hipStream_t s) {
 // the other layouts for the length (2 at 512, 3 at 1024, 1 otherwise)
 template <int N>
 static hipError_t row_pl(int mode, int mul, const RowArgs& a, hipStream_t s) {
-  const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0) | (a.aux ? STORE_AUX : 0);
   const bool shared_t = a.t && !a.t_index && a.t_group >= a.nbatch;
+  // per-wave transmission slots (the serving engine's batches of several
+  // jobs) take the loop's slot-following form, from 256 up
+  const bool slots = a.t && !shared_t && N >= 256;
+  const int fl = (a.prune_in ? PRUNE_IN : 0) | (a.prune_out ? PRUNE_OUT : 0) | (a.aux ? STORE_AUX : 0) |
+                 (slots ? T_SLOTS : 0);
   // a batch narrower than two waves per line would leave most lines idle
-  if (mode != ROW_INV_MUL_FWD || mul != RM_T || !shared_t || a.ny > 65535 || a.nbatch < 2 * RowPL<N>::LINES) {
+  if (mode != ROW_INV_MUL_FWD || mul != RM_T || !(shared_t || slots) || a.ny > 65535 ||
+      a.nbatch < 2 * RowPL<N>::LINES) {
     if constexpr (N == 512) return row_nv<N, 2>(mode, mul, a, s);
     else if constexpr (N == 1024) return row_nv<N, 3>(mode, mul, a, s);
     else return row_nv<N, 1>(mode, mul, a, s);
@@ -923,7 +948,16 @@ static hipError_t row_pl(int mode, int mul, const RowArgs& a, hipStream_t s) {
   else if (fl == PRUNE_OUT) hipLaunchKernelGGL((k_row_pl<N, PRUNE_OUT>), grid, block, 0, s, a, per_line);
   else if (fl == PRUNE_IN) hipLaunchKernelGGL((k_row_pl<N, PRUNE_IN>), grid, block, 0, s, a, per_line);
   else if (fl == 0) hipLaunchKernelGGL((k_row_pl<N, 0>), grid, block, 0, s, a, per_line);
-  else return row_nv<N, 1>(mode, mul, a, s);
+  else if constexpr (N >= 256) {
+    if (fl == (PRUNE_IN | PRUNE_OUT | T_SLOTS))
+      hipLaunchKernelGGL((k_row_pl<N, PRUNE_IN | PRUNE_OUT | T_SLOTS>), grid, block, 0, s, a, per_line);
+    else if (fl == (PRUNE_OUT | T_SLOTS))
+      hipLaunchKernelGGL((k_row_pl<N, PRUNE_OUT | T_SLOTS>), grid, block, 0, s, a, per_line);
+    else if (fl == T_SLOTS) hipLaunchKernelGGL((k_row_pl<N, T_SLOTS>), grid, block, 0, s, a, per_line);
+    else return row_nv<N, 1>(mode, mul, a, s);
+  } else {
+    return row_nv<N, 1>(mode, mul, a, s);
+  }
   return hipGetLastError();
 }
 
